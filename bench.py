@@ -1,0 +1,189 @@
+"""Benchmark: FP32 shot-gradients/s of the elastic velocity-strain FWI
+gradient path at 152x500, nt=3000 (25 moment sources, 500 receivers, fd8,
+free surface, 20-cell C-PML), on one MI355X.
+
+A step = one misfit+gradient evaluation of all 25 shots: batched forward
+with strain history, residual prep, adjoint with the exact imaging condition
+fused into each reverse step, shot reduction and chain rule to Vs.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+NZ, NX, PAD, FD, NT, DT, FC = 152, 500, 20, 8, 3000, 1.5e-3, 8.0
+DX = DZ = 20.0
+NSHOT, NREC = 25, 500
+HBM_PEAK = 8.0e12
+
+
+def algorithmic_bytes_per_shot_gradient(s=4, k=1, nt=NT):
+    """Minimal single-pass traffic: every array touched once per step.
+    forward step: s (13 N + 10 N_pml + 3 N_p + 2 nrec)
+    adjoint step: s (13 N + 10 N_pml + 3 N_p + 2 k N_p + 2 nrec)."""
+    nz_t, nx_t = NZ + PAD, NX + 2 * PAD
+    N = nz_t * nx_t
+    Np = NZ * NX
+    Npml = nz_t * 2 * PAD + PAD * nx_t     # cells of the x- and z-strips
+    fwd = s * (13 * N + 10 * Npml + 3 * Np + 2 * NREC)
+    adj = s * (13 * N + 10 * Npml + 3 * Np + 2 * k * Np + 2 * NREC)
+    return nt * (fwd + adj), fwd, adj
+
+
+def dominant_kernel_bytes(s=4, k=1):
+    """Algorithmic bytes of one launch of the velocity-adjoint kernel for all
+    shots: reads U_a, V_a, B, L2M, M (5 N), read+write exx_a, ezz_a, exz_a
+    (6 N), read+write of its 4 C-PML strips (8 N_strip), the 3 history
+    frames (3 N_p) and k gradient accumulators (2 k N_p)."""
+    nz_t, nx_t = NZ + PAD, NX + 2 * PAD
+    N = nz_t * nx_t
+    Np = NZ * NX
+    nxs = nz_t * 2 * PAD
+    nzs = PAD * nx_t
+    return NSHOT * s * (11 * N + 2 * 2 * (nxs + nzs) + 3 * Np + 2 * k * Np)
+
+
+def survey(seed=0):
+    from stfwi_amd.models import layered_model, smooth_initial, ricker
+    vs_t, vp_t, rho_t = layered_model(NZ, NX, seed=seed)
+    vs0, vp0, rho0 = smooth_initial(vs_t, vp_t, rho_t, 10.0)
+    src_x = 10 + 20 * np.arange(NSHOT)
+    src_z = np.full(NSHOT, 2)
+    rec_x = np.arange(NREC)
+    rec_z = np.zeros(NREC, int)
+    wav = ricker(NT, DT, FC)
+    return dict(true=(vs_t, vp_t, rho_t), init=(vs0, vp0, rho0), src_x=src_x, src_z=src_z,
+                rec_x=rec_x, rec_z=rec_z, wav=wav)
+
+
+def cpu_baseline(sv, budget_s=15.0):
+    """The oracle (C restatement, FP32, OpenMP on the host cores) on a bounded
+    sample of the workload: one shot's forward + exact adjoint for as many
+    steps as fit in the budget, scaled to shot-gradients/s."""
+    import oracle
+    from oracle import signal as osig
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except Exception:
+        pass
+    cores = min(cores, 16)
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    vs0, vp0, rho0 = sv["init"]
+    nt_s = 100
+    g = oracle.Geom(NZ, NX, PAD, True, FD, nt_s, DX, DZ, DT, FC)
+    t0 = time.perf_counter()
+    reps = 0
+    while True:
+        rec, t = oracle.forward(g, vs0, vp0, rho0, sv["src_x"][0], sv["src_z"][0], sv["wav"][:nt_s],
+                                sv["rec_x"], sv["rec_z"], "moment", np.float32, ("e", "dxu0"))
+        ra = np.zeros_like(rec)
+        ra[2] = rec[2]
+        oracle.adjoint(g, vs0, vp0, rho0, sv["src_x"][0], sv["src_z"][0], ra, sv["rec_x"], sv["rec_z"],
+                       "moment", sv["wav"][:nt_s], "exact", t, None, np.float32)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el > budget_s:
+            break
+    per_step = el / (reps * nt_s)
+    return {"value": 1.0 / (per_step * NT), "unit": "shot-gradients/s", "cores": cores, "kind": "port",
+            "sample": f"{reps}x one shot, {nt_s} forward+adjoint steps (exact imaging, FP32) of the "
+                      f"152x500 fd8 workload, scaled to nt={NT}; {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--imaging", default="exact", choices=["exact", "zhou"])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU design "
+              "(shot-parallel with one all-reduce per gradient) is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from stfwi_amd import Plan
+    from stfwi_amd import _abi
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    sv = survey(0)
+    f32 = torch.float32
+    plan = Plan(NZ, NX, DX, DZ, DT, FC, PAD, fd_order=FD, free_surface=True, dtype=f32, n_shots_max=NSHOT,
+                nt_max=NT, nrec_max=NREC, imaging=args.imaging, grad="vs", store_history=True)
+    # observed data on the true model
+    plan.set_model(*sv["true"])
+    plan.set_shots(sv["src_x"], sv["src_z"], "moment", NT, sv["rec_x"], sv["rec_z"], sv["wav"])
+    obs = plan.forward(channels=(2,))["rec"][:, 2].contiguous()
+    scale = float(1.0 / (torch.mean(obs.double() ** 2) + 1e-45))
+    vs0, vp0, rho0 = (torch.as_tensor(a, dtype=f32, device=dev) for a in sv["init"])
+
+    def step():
+        plan.set_model(vs0, vp0, rho0)
+        rec = plan.forward(channels=(2,))["rec"]
+        loss, radj = plan.residual(rec, obs, 2, None, 0, scale, NSHOT)
+        plan.adjoint(radj, want_src=False)
+        g = plan.finalize(vs0, vp0, rho0)
+        return loss, g
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loss, g = step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ms = el / args.steps * 1e3
+    value = NSHOT / (el / args.steps)
+    ok = bool(torch.isfinite(g).all()) and bool(torch.isfinite(loss).all())
+
+    # dominant kernel timed live: the velocity-adjoint launches of one
+    # reverse time loop, bracketed with HIP events on the plan's stream
+    L = _abi.lib()
+    kms = C.c_double()
+    kern = {}
+    if hasattr(L, "stfwi_time_kernel"):
+        L.stfwi_time_kernel.restype = C.c_int
+        L.stfwi_time_kernel.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_double)]
+        reps = 200
+        _abi.check(L.stfwi_time_kernel(plan._h, 3, reps, C.c_void_p(torch.cuda.current_stream().cuda_stream),
+                                       C.byref(kms)))
+        avg_s = kms.value / 1e3 / reps
+        kb = dominant_kernel_bytes(4, 1)
+        kern = {"bound": "hbm", "achieved": kb / avg_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": kb / avg_s / HBM_PEAK, "traffic": None, "kernel": "k_adj_vel<float,8>",
+                "avg_launch_us": avg_s * 1e6, "algorithmic_bytes_per_launch": kb}
+    tot, _, _ = algorithmic_bytes_per_shot_gradient(4, 1)
+    path = {"algorithmic_GB_per_shot_gradient": tot / 1e9, "achieved_GB_s": value * tot / 1e9,
+            "frac_of_hbm_peak": value * tot / HBM_PEAK}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(sv)
+    out = {"metric": "shot-gradients/s at 152x500 nt=3000 (%HBM roofline)", "value": value,
+           "unit": "shot-gradients/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic (seeded layered model, Ricker 8 Hz)",
+           "config": {"workload": "25-shot gradient, 152x500 fd8 free surface pad 20, nt=3000, "
+                                  "500 receivers, moment sources, exact imaging, Vs",
+                      "shots": NSHOT, "nt": NT, "grid": [NZ, NX], "imaging": args.imaging},
+           "roofline": kern or None, "path_roofline": path, "cpu_baseline": cpu,
+           "finite": ok, "loss": float(loss.double().sum())}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,157 @@
+/*
+ * stfwi — MI355X (gfx950) elastic velocity-strain FWI gradient path.
+ *
+ * C-ABI of libstfwi.so.  All array arguments are DEVICE pointers owned by the
+ * caller (the Python layer passes torch tensor data pointers), except where a
+ * comment says "host".  Every call is asynchronous on the given hipStream_t
+ * (passed as void*; NULL = the default stream) unless documented otherwise.
+ * Functions return 0 on success and a negative stfwi_status on failure;
+ * stfwi_last_error() then holds a message (thread-local).  No C++ exception
+ * crosses this boundary.  A plan is not reentrant: use one plan per host
+ * thread and stream.
+ *
+ * The reference (x-repos/stFWI) has no native boundary; its interface is the
+ * Python call signatures cited per entry point below.
+ */
+#ifndef STFWI_H
+#define STFWI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define STFWI_ABI_VERSION 1
+
+typedef enum {
+    STFWI_OK = 0,
+    STFWI_ERR_INVALID = -1,     /* bad argument */
+    STFWI_ERR_UNSUPPORTED = -2, /* fd_order / source type / mode not supported */
+    STFWI_ERR_HIP = -3,         /* a HIP runtime call failed */
+    STFWI_ERR_OOM = -4,         /* device allocation failed */
+    STFWI_ERR_STATE = -5        /* call out of order (e.g. adjoint before forward) */
+} stfwi_status;
+
+enum { STFWI_F32 = 0, STFWI_F64 = 1 };
+enum { STFWI_SRC_FORCE = 0, STFWI_SRC_MOMENT = 1 };
+enum { STFWI_IMAGING_NONE = 0, STFWI_IMAGING_ZHOU = 1, STFWI_IMAGING_EXACT = 2 };
+/* gradient parameter sets (the reference's invert='vs' | 'vs_vp' | 'all') */
+enum { STFWI_GRAD_VS = 1, STFWI_GRAD_VS_VP = 2, STFWI_GRAD_ALL = 3 };
+/* receiver channels, in the order of the reference's rec tuple
+ * (vx, vz, exx, ezz), fwi/forward.py:465 */
+enum { STFWI_CH_VX = 1, STFWI_CH_VZ = 2, STFWI_CH_EX = 4, STFWI_CH_EZ = 8 };
+/* full-field snapshot selection, fwi/forward.py:450-460 ('*_full') */
+enum { STFWI_SNAP_VX = 1, STFWI_SNAP_VZ = 2, STFWI_SNAP_EX = 4, STFWI_SNAP_EZ = 8, STFWI_SNAP_ES = 16 };
+
+/* Grid, time axis and sizing.  Replaces the static arguments of
+ * forward_jax (fwi/forward.py:119-123) and adjoint_jax
+ * (fwi/adjoint_jax_kernel.py:61-69). */
+typedef struct stfwi_config {
+    int nz_dom, nx_dom;   /* physical grid (rows = z) */
+    int pad;              /* C-PML thickness in cells */
+    int free_surface;     /* 1: free surface on top, PML on 3 sides */
+    int fd_order;         /* 2, 4 or 8 */
+    double dx, dz, dt;
+    double fc;            /* PML alpha tuning frequency */
+    int dtype;            /* STFWI_F32 or STFWI_F64 */
+    int n_shots_max;      /* shots per batched launch */
+    int nt_max;           /* longest record */
+    int nrec_max;         /* most receivers of any shot */
+    int imaging;          /* STFWI_IMAGING_* (history storage is sized for it) */
+    int grad_params;      /* STFWI_GRAD_* */
+    int store_history;    /* 1: keep the strain history for the adjoint */
+} stfwi_config;
+
+/* One shot.  Indices are physical-domain (the reference's src_x, src_z,
+ * rec_x, rec_z, fwi/forward.py:240-242). */
+typedef struct stfwi_shot {
+    int src_x, src_z;
+    int src_type;         /* STFWI_SRC_* */
+    int nt;               /* record length, <= nt_max */
+    int rec_offset;       /* first receiver in the rec_x/rec_z host arrays */
+    int nrec;             /* <= nrec_max */
+} stfwi_shot;
+
+typedef struct stfwi_plan stfwi_plan;
+
+int stfwi_abi_version(void);
+const char *stfwi_last_error(void);
+
+/* Allocate device workspace (state, PML strips, history).  Host call,
+ * synchronous.  Replaces the per-call set-up of forward_jax
+ * (fwi/forward.py:167-247). */
+int stfwi_plan_create(const stfwi_config *cfg, stfwi_plan **out);
+int stfwi_plan_destroy(stfwi_plan *plan);
+/* bytes of device memory the plan holds (host call) */
+long long stfwi_plan_bytes(const stfwi_plan *plan);
+
+/* Model: vs, vp, rho (nz_dom x nx_dom, plan dtype).  Builds the padded
+ * moduli, buoyancy and the 1-D C-PML profiles on the device
+ * (fwi/forward.py:193-238, fwi/forward.py:35-104).  b_img (optional, may be
+ * NULL) is the physical buoyancy the Zhou imaging uses (fwi/imaging.py:113-114
+ * computes it in the model's storage dtype); NULL means 1/rho. */
+int stfwi_set_model(stfwi_plan *plan, const void *vs, const void *vp, const void *rho,
+                    const void *b_img, void *stream);
+
+/* Shots: host table of n_shots, host receiver index arrays, device wavelets
+ * (n_shots x nt_max, plan dtype).  Host call: uploads the table and the
+ * collision-merged receiver cells (duplicates accumulate, as the reference's
+ * .at[].add does, fwi/adjoint_jax_kernel.py:203-206). */
+int stfwi_set_shots(stfwi_plan *plan, const stfwi_shot *shots, int n_shots,
+                    const int *rec_x, const int *rec_z, const void *wavelets, void *stream);
+
+/* Forward modelling of all shots in one batched time loop
+ * (forward_jax, fwi/forward.py:119-501).
+ *   rec_out   n_shots x 4 x nt_max x nrec_max, channels vx vz exx ezz (or NULL)
+ *   rec_mask  STFWI_CH_* bits to record
+ *   illum     n_shots x nz_t x nx_t accumulators of sum_t field^2 (or NULL),
+ *             illum_comp = 0..3 (vx, vz, exx, ezz) as in
+ *             fwi/illumination.py:22-34; the caller zeroes it
+ *   snap      n_shots x popcount(snap_mask) x nt_max x nz_t x nx_t full
+ *             padded fields after each step (or NULL), STFWI_SNAP_* bits
+ * With store_history the plan keeps the strains needed by stfwi_adjoint. */
+int stfwi_forward(stfwi_plan *plan, void *rec_out, int rec_mask, void *illum, int illum_comp,
+                  void *snap, int snap_mask, void *stream);
+
+/* Hand-coded adjoint of all shots (adjoint_jax,
+ * fwi/adjoint_jax_kernel.py:65-483) with the imaging condition of the plan
+ * fused into each reverse step.
+ *   rec_adj   n_shots x 4 x nt_max x nrec_max receiver adjoints
+ *   src_adj   n_shots x nt_max source-wavelet adjoint (or NULL)
+ *   snap      n_shots x popcount(snap_mask) x nt_max x nz_t x nx_t adjoint
+ *             carry-out fields (or NULL; fwi/adjoint_jax_kernel.py:470-476)
+ * Gradients stay in the plan until stfwi_gradient_finalize. */
+int stfwi_adjoint(stfwi_plan *plan, const void *rec_adj, void *src_adj, void *snap, int snap_mask,
+                  void *stream);
+
+/* Sum the per-shot accumulators and chain them to (Vs, Vp, rho) on the
+ * physical grid (fwi/imaging.py:164-167 for Zhou; the exact chain for
+ * 'ad').  vs/vp/rho are the model arrays whose dtype (chain_dtype,
+ * STFWI_F32/F64) the chain factors are formed in, as the reference forms
+ * 2*rho*Vs in the model's own dtype.  g_out: 3 x nz_dom x nx_dom in the plan
+ * dtype; parameters outside grad_params are written as zeros. */
+int stfwi_gradient_finalize(stfwi_plan *plan, const void *vs, const void *vp, const void *rho,
+                            int chain_dtype, void *g_out, void *stream);
+
+/* Residual prep of all shots (fwi/gradient.py:139-156, fwi/filters.py):
+ * taper the last n_taper samples, filter with the DF-I biquad cascade sos
+ * (host, n_sos x 6, may be NULL), misfit (scale/n_shots) mean((F T pred -
+ * F T obs)^2) per shot into loss_out (n_shots, plan dtype), and the receiver
+ * cotangent into channel `component` of rec_adj (other channels zeroed).
+ *   pred      n_shots x 4 x nt_max x nrec_max (the forward's rec_out)
+ *   obs       n_shots x nt_max x nrec_max
+ *   n_shots_norm  the n_shots of the reference's normalisation */
+int stfwi_residual(stfwi_plan *plan, const void *pred, const void *obs, int component,
+                   const double *sos, int n_sos, int n_taper, double scale, int n_shots_norm,
+                   void *loss_out, void *rec_adj, void *stream);
+
+/* Diagnostics (no reference counterpart): time `reps` back-to-back launches
+ * of one time-step kernel over all shots of the plan (which: 0 forward
+ * velocity, 1 forward strain, 2 adjoint strain, 3 adjoint velocity with
+ * imaging) with HIP events on `stream`; *ms receives the total.  Host call,
+ * synchronous.  It overwrites the plan's wavefield state. */
+int stfwi_time_kernel(stfwi_plan *plan, int which, int reps, void *stream, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

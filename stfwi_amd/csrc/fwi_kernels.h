@@ -1,0 +1,965 @@
+// Device kernels of the elastic velocity-strain FWI gradient path (gfx950).
+//
+// Layout (all fields row-major, z = row):
+//   * every wavefield/modulus buffer is (nz + 8) x ld with ld >= nx + 8 and a
+//     4-row zero margin above and below; the row gap [nx, ld) is never
+//     written, so stencil loads at +-4 cells stay inside the allocation and
+//     read zeros there.  `origin` = 4*ld is the offset of cell (0,0).
+//   * C-PML memories exist only in the PML strips: x-strips are nz x 2*pad
+//     (left pad columns, then right pad columns), z-strips are zrows x nx
+//     (bottom pad rows first, then the top pad rows when there is no free
+//     surface).  Outside the strips the reference's recursion is an exact
+//     no-op (b = 1, a = 0, kappa = 1, psi stays 0).
+//   * histories keep the physical cells only: [shot][t][field][nz_dom*nx_dom].
+//
+// Every derivative keeps the reference's per-term output ranges
+// (fwi/forward.py:256-441): term 1 covers its own slice, term k >= 2 the
+// range where both reads are in bounds ([k-1, n-1-k] for D+, [k, n-k] for
+// D-).  The adjoint kernels are the exact transposes written as gathers:
+// every output cell sums the contributions that the reference scatters into
+// it (fwi/adjoint_jax_kernel.py:208-455), so no atomics are needed.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace stfwi {
+
+typedef long long i64;
+
+template <int FD> struct FDC;
+template <> struct FDC<2> { static constexpr int NK = 1; };
+template <> struct FDC<4> { static constexpr int NK = 2; };
+template <> struct FDC<8> { static constexpr int NK = 4; };
+
+// staggered-grid coefficients (fwi/forward.py:177-182)
+template <int FD>
+__host__ __device__ constexpr double fdc(int k)
+{
+    return FD == 8 ? (k == 1 ? 1225.0 / 1024.0 : k == 2 ? -245.0 / 3072.0 : k == 3 ? 49.0 / 5120.0 : -5.0 / 7168.0)
+         : FD == 4 ? (k == 1 ? 9.0 / 8.0 : -1.0 / 24.0)
+                   : 1.0;
+}
+
+struct ShotDev {
+    int sx, sz;        // padded-grid source cell
+    int type;          // 0 force, 1 moment
+    int nt;            // record length
+    int rec_off, nrec; // receivers [rec_off, rec_off + nrec) of rec_cell
+    int uc_off, nuc;   // merged receiver cells [uc_off, uc_off + nuc)
+};
+
+template <typename T>
+struct Geo {
+    int nz, nx, ld;          // padded grid and field pitch
+    int z0, pad, nzd, nxd;   // physical window
+    int fs, zrows;           // free surface; rows of a z-strip
+    i64 fstride;             // elements per field buffer
+    i64 origin;              // offset of cell (0,0) in a field buffer
+    i64 np;                  // physical cells
+    T dt, dx, dz, dtdx, dtdz, hdtdx, hdtdz, dzdx, fsrc;
+};
+
+template <typename T>
+__device__ __forceinline__ int xstrip(const Geo<T> &g, int r, int c)
+{
+    if (c < g.pad) return r * 2 * g.pad + c;
+    if (c >= g.nx - g.pad) return r * 2 * g.pad + c - g.nx + 2 * g.pad;
+    return -1;
+}
+
+template <typename T>
+__device__ __forceinline__ int zstrip(const Geo<T> &g, int r, int c)
+{
+    if (r >= g.nz - g.pad) return (r - (g.nz - g.pad)) * g.nx + c;
+    if (!g.fs && r < g.pad) return (g.pad + r) * g.nx + c;
+    return -1;
+}
+
+// profile layout per axis: kappa[n], b[n], a[n], q[n] with q = 1/kappa + a
+template <typename T>
+__device__ __forceinline__ T pml_fwd(T D, T *psi, const T *prof, int j, int n)
+{
+    const T pn = prof[n + j] * (*psi) + prof[2 * n + j] * D;   // psi' = b psi + a D
+    *psi = pn;
+    return D / prof[j] + pn;                                   // D' = D / kappa + psi'
+}
+
+template <typename T>
+__device__ __forceinline__ T pmlT_val(T G, T psi_old, const T *prof, int j, int n)
+{
+    return prof[3 * n + j] * G + prof[2 * n + j] * psi_old;    // (1/kappa + a) G + a psi
+}
+
+template <typename T>
+__device__ __forceinline__ T pmlT_psi(T G, T psi_old, const T *prof, int j, int n)
+{
+    return prof[n + j] * (G + psi_old);                        // b (G + psi)
+}
+
+// ---------------------------------------------------------------------------
+// model set-up (fwi/forward.py:193-238)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_model(Geo<T> g, const T *__restrict__ vs, const T *__restrict__ vp,
+                        const T *__restrict__ rho, const T *__restrict__ bimg_phys,
+                        T *L2M, T *M, T *B, T *Bimg, T *Lr0)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (c >= g.nx || r >= g.nz) return;
+    // nearest-neighbour extension into the PML == clamped indexing
+    int rr = r - g.z0, cc = c - g.pad;
+    rr = rr < 0 ? 0 : (rr >= g.nzd ? g.nzd - 1 : rr);
+    cc = cc < 0 ? 0 : (cc >= g.nxd ? g.nxd - 1 : cc);
+    const i64 ip = (i64)rr * g.nxd + cc;
+    const T s = vs[ip], p = vp[ip], d = rho[ip];
+    const T m = d * (s * s), l2m = d * (p * p);
+    const i64 q = g.origin + (i64)r * g.ld + c;
+    M[q] = m;
+    L2M[q] = l2m;
+    B[q] = T(1) / d;
+    Bimg[q] = bimg_phys ? bimg_phys[ip] : T(1) / d;
+    if (r == 0) Lr0[c] = -(l2m - T(2) * m) / l2m;   // L_ratio = -lambda / (lambda + 2 mu)
+}
+
+template <typename T>
+__global__ void k_vpmax(const T *__restrict__ vp, i64 n, double *out)
+{
+    __shared__ double red[256];
+    double m = -1e300;
+    for (i64 i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, (double)vp[i]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+// C-PML profiles (fwi/forward.py:35-104): N = 2, Rc = 1e-3, L = pad*dx for
+// both axes, d = d0 dist^2, kappa = 1 + dist^2, alpha = pi fc (1 - dist).
+template <typename T>
+__global__ void k_profiles(int nz, int nx, int pad, int fs, int nzd, double dx, double dt, double fc,
+                           const double *vpmax, T *profx, T *profz)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const double Lp = pad * dx;
+    const double d0 = -(2 + 1) * (*vpmax) * log(1e-3) / (2 * Lp);
+    for (int axis = 0; axis < 2; ++axis) {
+        const int n = axis == 0 ? nx : nz;
+        if (j >= n) continue;
+        int i = -1;   // profile index of this column / row
+        if (axis == 0) {
+            if (j < pad) i = pad - 1 - j;
+            else if (j >= nx - pad) i = j - (nx - pad);
+        } else {
+            const int bottom = fs ? nzd : pad + nzd;
+            if (j >= bottom && j < bottom + pad) i = j - bottom;
+            else if (!fs && j < pad) i = pad - 1 - j;
+        }
+        double d = 0, k = 1, al = 0;
+        if (i >= 0) {
+            const double dist = (i + 1) * dx / Lp;
+            d = d0 * (dist * dist);
+            k = 1.0 + (2.0 - 1.0) * (dist * dist);
+            al = M_PI * fc * (1.0 - dist);
+        }
+        const double b = exp(-(d / k + al) * dt);
+        const double den = k * (d + k * al);
+        const double a = den > 1e-10 ? d * (b - 1.0) / den : 0.0;
+        T *p = axis == 0 ? profx : profz;
+        p[j] = (T)k;
+        p[n + j] = (T)b;
+        p[2 * n + j] = (T)a;
+        p[3 * n + j] = T(1) / (T)k + (T)a;
+    }
+}
+
+template <typename T>
+__global__ void k_fill(T *p, i64 n, T v)
+{
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// forward
+// ---------------------------------------------------------------------------
+template <typename T>
+struct FwdArgs {
+    Geo<T> g;
+    const ShotDev *shots;
+    const T *L2M, *M, *B, *Lr0;      // moduli buffers (origin applied by the kernel)
+    T *state;                        // per shot: U V exx ezz exz
+    i64 sstride;
+    T *psix, *psiz;                  // per shot: 5 x-strip / 5 z-strip arrays
+    i64 psix_stride, psiz_stride, psix_n, psiz_n;
+    const T *profx, *profz;
+    const T *wav;                    // [shot][nt_max]
+    int nt_max;
+    T *rec;                          // [shot][4][nt_max][nrec_max]
+    int nrec_max, rec_mask;
+    const i64 *rec_cell;             // cell offset r*ld + c of every receiver
+    T *hist;                         // [shot][nt_max][3][np] strains
+    T *hist2;                        // [shot][nt_max][2][np]: 1 = dU/dB,dV/dB  2 = U,V
+    int hist2_mode;
+    T *dxu0;                         // [shot][nt_max][nx]
+    T *illum;                        // [shot][nz*nx]
+    int illum_comp;
+    T *snap;                         // [shot][nsnap][nt_max][nz*nx]
+    int snap_mask, nsnap;
+    int tiles_x, tiles_z, ntiles;
+};
+
+#define FWD_TILE_X 64
+#define FWD_TILE_Z 4
+
+// velocity half-step (fwi/forward.py:253-367) + exx/ezz receivers of the
+// previous step (strains are read-only here)
+template <typename T, int FD>
+__global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
+{
+    const Geo<T> &g = a.g;
+    const int s = blockIdx.z;
+    const ShotDev sh = a.shots[s];
+    if ((int)blockIdx.x >= a.ntiles) {
+        const int j = ((int)blockIdx.x - a.ntiles) * 256 + threadIdx.y * FWD_TILE_X + threadIdx.x;
+        const int itp = it - 1;
+        if (itp < 0 || itp >= sh.nt || j >= sh.nrec || !a.rec) return;
+        const i64 cell = g.origin + a.rec_cell[sh.rec_off + j];
+        const T *st = a.state + s * a.sstride;
+        T *ro = a.rec + (i64)s * 4 * a.nt_max * a.nrec_max + (i64)itp * a.nrec_max + j;
+        if (a.rec_mask & 4) ro[(i64)2 * a.nt_max * a.nrec_max] = st[2 * g.fstride + cell];
+        if (a.rec_mask & 8) ro[(i64)3 * a.nt_max * a.nrec_max] = st[3 * g.fstride + cell];
+        return;
+    }
+    if (it >= sh.nt) return;
+    const int tz = blockIdx.x / a.tiles_x, tx = blockIdx.x - tz * a.tiles_x;
+    const int r = tz * FWD_TILE_Z + threadIdx.y, c = tx * FWD_TILE_X + threadIdx.x;
+    if (r > g.nz - 2 || c < 1 || c > g.nx - 2) return;
+    constexpr int NK = FDC<FD>::NK;
+    const int ld = g.ld, nz = g.nz, nx = g.nx;
+    T *st = a.state + s * a.sstride + g.origin;
+    T *__restrict__ U = st;
+    T *__restrict__ V = st + g.fstride;
+    const T *__restrict__ ex = st + 2 * g.fstride;
+    const T *__restrict__ ez = st + 3 * g.fstride;
+    const T *__restrict__ es = st + 4 * g.fstride;
+    const T *__restrict__ L2M = a.L2M + g.origin;
+    const T *__restrict__ M = a.M + g.origin;
+    const T *__restrict__ B = a.B + g.origin;
+    const i64 q = (i64)r * ld + c;
+    auto P1 = [&](i64 p) { return L2M[p] * ex[p]; };                       // (lambda+2mu) exx
+    auto P2 = [&](i64 p) { return (L2M[p] - T(2) * M[p]) * ez[p]; };       // lambda ezz
+    auto P3 = [&](i64 p) { return T(2) * M[p] * es[p]; };                  // 2 mu exz
+    auto P4 = [&](i64 p) { return L2M[p] * ez[p]; };
+    auto P5 = [&](i64 p) { return (L2M[p] - T(2) * M[p]) * ex[p]; };
+    T D1 = 0, D2 = 0, D3 = 0, D4 = 0, D5 = 0, D6 = 0;
+#pragma unroll
+    for (int k = 1; k <= NK; ++k) {
+        const T ck = (T)fdc<FD>(k);
+        // D+x, term range [k-1, nx-1-k] (term 1: [0, nx-2], always true here)
+        if (c >= k - 1 && c <= nx - 1 - k) {
+            D1 += ck * (P1(q + k) - P1(q - k + 1));
+            D2 += ck * (P2(q + k) - P2(q - k + 1));
+        }
+        // D-z of 2 mu exz, term 1 rows [1, nz-2], term k rows [k, nz-k]
+        if (k == 1 ? (r >= 1 && r <= nz - 2) : (r >= k && r <= nz - k))
+            D3 += ck * (P3(q + (i64)(k - 1) * ld) - P3(q - (i64)k * ld));
+        // D-x of 2 mu exz, term 1 cols [1, nx-2], term k cols [k, nx-k]
+        if (k == 1 || (c >= k && c <= nx - k)) D4 += ck * (P3(q + k - 1) - P3(q - k));
+        // D+z, term range [k-1, nz-1-k]
+        if (r >= k - 1 && r <= nz - 1 - k) {
+            D5 += ck * (P4(q + (i64)k * ld) - P4(q - (i64)(k - 1) * ld));
+            D6 += ck * (P5(q + (i64)k * ld) - P5(q - (i64)(k - 1) * ld));
+        }
+    }
+    if (g.fs && r == 0) D3 = T(2 * fdc<FD>(1)) * P3(q);   // image of sigma_xz at the surface
+    const int ix = xstrip(g, r, c), iz = zstrip(g, r, c);
+    if (ix >= 0) {
+        T *px = a.psix + s * a.psix_stride + ix;
+        D1 = pml_fwd(D1, px + 0 * a.psix_n, a.profx, c, nx);
+        D2 = pml_fwd(D2, px + 1 * a.psix_n, a.profx, c, nx);
+        D4 = pml_fwd(D4, px + 2 * a.psix_n, a.profx, c, nx);
+    }
+    if (iz >= 0) {
+        T *pz = a.psiz + s * a.psiz_stride + iz;
+        D3 = pml_fwd(D3, pz + 0 * a.psiz_n, a.profz, r, nz);
+        D5 = pml_fwd(D5, pz + 1 * a.psiz_n, a.profz, r, nz);
+        D6 = pml_fwd(D6, pz + 2 * a.psiz_n, a.profz, r, nz);
+    }
+    const T b = B[q];
+    U[q] = U[q] + g.dtdx * b * D1 + g.dtdx * b * D2 + g.dtdz * b * D3;
+    T v = V[q] + g.dtdx * b * D4 + g.dtdz * b * D5 + g.dtdz * b * D6;
+    T sv = 0;
+    if (sh.type == 0 && r == sh.sz && c == sh.sx) {
+        const T w = a.wav[(i64)s * a.nt_max + it];
+        v += g.fsrc * b * w;
+        sv = g.fsrc * w;
+    }
+    V[q] = v;
+    if (a.hist2_mode == 1) {
+        const int rp = r - g.z0, cp = c - g.pad;
+        if (rp >= 0 && rp < g.nzd && cp >= 0 && cp < g.nxd) {
+            T *h = a.hist2 + ((i64)s * a.nt_max + it) * 2 * g.np + (i64)rp * g.nxd + cp;
+            h[0] = g.dtdx * D1 + g.dtdx * D2 + g.dtdz * D3;
+            h[g.np] = g.dtdx * D4 + g.dtdz * D5 + g.dtdz * D6 + sv;
+        }
+    }
+}
+
+// strain half-step (fwi/forward.py:369-444) + vx/vz receivers, history,
+// illumination energy, snapshots
+template <typename T, int FD>
+__global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
+{
+    const Geo<T> &g = a.g;
+    const int s = blockIdx.z;
+    const ShotDev sh = a.shots[s];
+    if (it >= sh.nt) return;
+    if ((int)blockIdx.x >= a.ntiles) {
+        const int j = ((int)blockIdx.x - a.ntiles) * 256 + threadIdx.y * FWD_TILE_X + threadIdx.x;
+        if (j >= sh.nrec || !a.rec) return;
+        const i64 cell = g.origin + a.rec_cell[sh.rec_off + j];
+        const T *st = a.state + s * a.sstride;
+        T *ro = a.rec + (i64)s * 4 * a.nt_max * a.nrec_max + (i64)it * a.nrec_max + j;
+        if (a.rec_mask & 1) ro[0] = st[cell];
+        if (a.rec_mask & 2) ro[(i64)a.nt_max * a.nrec_max] = st[g.fstride + cell];
+        return;
+    }
+    const int tz = blockIdx.x / a.tiles_x, tx = blockIdx.x - tz * a.tiles_x;
+    const int r = tz * FWD_TILE_Z + threadIdx.y, c = tx * FWD_TILE_X + threadIdx.x;
+    if (r >= g.nz || c >= g.nx) return;
+    constexpr int NK = FDC<FD>::NK;
+    const int ld = g.ld, nz = g.nz, nx = g.nx;
+    T *st = a.state + s * a.sstride + g.origin;
+    const T *__restrict__ U = st;
+    const T *__restrict__ V = st + g.fstride;
+    T *__restrict__ ex = st + 2 * g.fstride;
+    T *__restrict__ ez = st + 3 * g.fstride;
+    T *__restrict__ es = st + 4 * g.fstride;
+    const i64 q = (i64)r * ld + c;
+    const int ix = xstrip(g, r, c), iz = zstrip(g, r, c);
+    T *px = a.psix + s * a.psix_stride + ix;
+    T *pz = a.psiz + s * a.psiz_stride + iz;
+    // exx: D-x U on columns [1, nx-1]
+    T dxu = 0;
+    if (c >= 1) {
+#pragma unroll
+        for (int k = 1; k <= NK; ++k)
+            if (k == 1 || (c >= k && c <= nx - k))
+                dxu += (T)fdc<FD>(k) * (U[q + k - 1] - U[q - k]);
+    }
+    if (ix >= 0) dxu = pml_fwd(dxu, px + 3 * a.psix_n, a.profx, c, nx);
+    T exn = c >= 1 ? ex[q] + g.dtdx * dxu : ex[q];
+    // ezz: D-z V on columns [0, nx-2]; free-surface row from the C-PML'd D-x U
+    T dzv = 0;
+    if (c <= nx - 2) {
+        if (g.fs && r == 0) {
+            dzv = a.Lr0[c] * dxu * g.dz / g.dx;
+        } else {
+#pragma unroll
+            for (int k = 1; k <= NK; ++k)
+                if (k == 1 ? r >= 1 : (r >= k && r <= nz - k))
+                    dzv += (T)fdc<FD>(k) * (V[q + (i64)(k - 1) * ld] - V[q - (i64)k * ld]);
+        }
+    }
+    if (iz >= 0) dzv = pml_fwd(dzv, pz + 3 * a.psiz_n, a.profz, r, nz);
+    T ezn = ez[q] + g.dtdz * dzv;
+    if (sh.type == 1 && r == sh.sz && c == sh.sx) {
+        const T lm = (a.L2M[g.origin + q] - T(2) * a.M[g.origin + q]) + a.M[g.origin + q];
+        const T stm = a.wav[(i64)s * a.nt_max + it] * g.dt / (T(2) * lm * g.dx * g.dz);
+        exn += stm;
+        ezn += stm;
+    }
+    ex[q] = exn;
+    ez[q] = ezn;
+    // exz on [0, nz-2] x [0, nx-2]
+    T esn = es[q];
+    if (r <= nz - 2 && c <= nx - 2) {
+        T dzu = 0, dxv = 0;
+#pragma unroll
+        for (int k = 1; k <= NK; ++k) {
+            const T ck = (T)fdc<FD>(k);
+            if (c >= 1 && r >= k - 1 && r <= nz - 1 - k) dzu += ck * (U[q + (i64)k * ld] - U[q - (i64)(k - 1) * ld]);
+            if (c >= k - 1 && c <= nx - 1 - k) dxv += ck * (V[q + k] - V[q - k + 1]);
+        }
+        if (iz >= 0) dzu = pml_fwd(dzu, pz + 4 * a.psiz_n, a.profz, r, nz);
+        if (ix >= 0) dxv = pml_fwd(dxv, px + 4 * a.psix_n, a.profx, c, nx);
+        esn = esn + T(0.5) * (g.dtdz * dzu + g.dtdx * dxv);
+        es[q] = esn;
+    }
+    if (a.dxu0 && r == 0) a.dxu0[((i64)s * a.nt_max + it) * nx + c] = dxu;
+    const int rp = r - g.z0, cp = c - g.pad;
+    if (a.hist && rp >= 0 && rp < g.nzd && cp >= 0 && cp < g.nxd) {
+        const i64 o = ((i64)s * a.nt_max + it) * 3 * g.np + (i64)rp * g.nxd + cp;
+        a.hist[o] = exn;
+        a.hist[o + g.np] = ezn;
+        a.hist[o + 2 * g.np] = esn;
+        if (a.hist2_mode == 2) {
+            T *h = a.hist2 + ((i64)s * a.nt_max + it) * 2 * g.np + (i64)rp * g.nxd + cp;
+            h[0] = U[q];
+            h[g.np] = V[q];
+        }
+    }
+    const i64 pc = (i64)r * nx + c;
+    if (a.illum) {
+        const T f = a.illum_comp == 0 ? U[q] : a.illum_comp == 1 ? V[q] : a.illum_comp == 2 ? exn : ezn;
+        a.illum[(i64)s * nz * nx + pc] += f * f;
+    }
+    if (a.snap) {
+        const T vals[5] = {U[q], V[q], exn, ezn, esn};
+        int k = 0;
+        for (int f = 0; f < 5; ++f)
+            if (a.snap_mask & (1 << f)) {
+                a.snap[(((i64)s * a.nsnap + k) * a.nt_max + it) * nz * nx + pc] = vals[f];
+                ++k;
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// adjoint
+// ---------------------------------------------------------------------------
+template <typename T>
+struct AdjArgs {
+    Geo<T> g;
+    const ShotDev *shots;
+    const T *L2M, *M, *B, *Bimg, *Lr0;
+    T *state;                          // per shot: U_a V_a exx_a ezz_a exz_a
+    i64 sstride;
+    const T *psix_o, *psiz_o;          // adjoint C-PML memories, previous step
+    T *psix_n, *psiz_n;                // adjoint C-PML memories, this step
+    i64 psix_stride, psiz_stride, psix_len, psiz_len;
+    const T *profx, *profz;
+    const T *wav;
+    int nt_max;
+    const T *rec_adj;                  // [shot][4][nt_max][nrec_max]
+    int nrec_max;
+    const i64 *uc_cell;                // merged receiver cells
+    const int *uc_ptr, *uc_rec;        // CSR of receiver indices per merged cell
+    T *src_adj;                        // [shot][nt_max]
+    int imaging, gparams, nacc;
+    const T *hist, *hist2;
+    const T *dxu0;
+    T *gacc;                           // [shot][nacc][np]
+    T *snap;
+    int snap_mask, nsnap;
+    int tiles_x, tiles_z, ntiles;
+};
+
+// receiver-adjoint injection at the start of reverse step it
+// (fwi/adjoint_jax_kernel.py:201-206); duplicates accumulate
+template <typename T>
+__global__ void k_adj_inject(AdjArgs<T> a, int it)
+{
+    const int s = blockIdx.z;
+    const ShotDev sh = a.shots[s];
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= sh.nt || u >= sh.nuc) return;
+    const int ui = sh.uc_off + u;
+    const i64 cell = a.g.origin + a.uc_cell[ui];
+    T *st = a.state + s * a.sstride;
+    const T *ra = a.rec_adj + (i64)s * 4 * a.nt_max * a.nrec_max + (i64)it * a.nrec_max;
+    for (int ch = 0; ch < 4; ++ch) {
+        T sum = 0;
+        for (int p = a.uc_ptr[ui]; p < a.uc_ptr[ui + 1]; ++p) sum += ra[(i64)ch * a.nt_max * a.nrec_max + a.uc_rec[p]];
+        st[ch * a.g.fstride + cell] += sum;
+    }
+}
+
+// adjoints of the four strain-update derivatives at any cell, after the
+// transposed C-PML (fwi/adjoint_jax_kernel.py:212-301)
+template <typename T>
+struct K4Src {
+    const Geo<T> &g;
+    const T *esa, *exa, *eza;
+    const T *pxo, *pzo;   // old psi base (per shot), arrays: x[0]=Dx_V x[1]=Dx_U, z[0]=Dz_U z[1]=Dz_V
+    i64 pxn, pzn;
+    const T *profx, *profz;
+    const T *Lr0;
+    __device__ __forceinline__ T GzU(int r, int c) const
+    {
+        return (r <= g.nz - 2 && c <= g.nx - 2) ? g.hdtdz * esa[(i64)r * g.ld + c] : T(0);
+    }
+    __device__ __forceinline__ T GxV(int r, int c) const
+    {
+        return (r <= g.nz - 2 && c <= g.nx - 2) ? g.hdtdx * esa[(i64)r * g.ld + c] : T(0);
+    }
+    __device__ __forceinline__ T GzV(int r, int c) const { return g.dtdz * eza[(i64)r * g.ld + c]; }
+    __device__ __forceinline__ T DzVa(int r, int c) const
+    {
+        const T G = GzV(r, c);
+        const int iz = zstrip(g, r, c);
+        return iz >= 0 ? pmlT_val(G, pzo[pzn + iz], profz, r, g.nz) : G;
+    }
+    __device__ __forceinline__ T GxU(int r, int c) const
+    {
+        T G = 0;
+        if (g.fs && r == 0 && c <= g.nx - 2) G = Lr0[c] * g.dzdx * DzVa(0, c);
+        if (c >= 1) G = G + g.dtdx * exa[(i64)r * g.ld + c];
+        return G;
+    }
+    __device__ __forceinline__ T DzUa(int r, int c) const
+    {
+        const T G = GzU(r, c);
+        const int iz = zstrip(g, r, c);
+        return iz >= 0 ? pmlT_val(G, pzo[iz], profz, r, g.nz) : G;
+    }
+    __device__ __forceinline__ T DxVa(int r, int c) const
+    {
+        const T G = GxV(r, c);
+        const int ix = xstrip(g, r, c);
+        return ix >= 0 ? pmlT_val(G, pxo[ix], profx, c, g.nx) : G;
+    }
+    __device__ __forceinline__ T DxUa(int r, int c) const
+    {
+        const T G = GxU(r, c);
+        const int ix = xstrip(g, r, c);
+        return ix >= 0 ? pmlT_val(G, pxo[pxn + ix], profx, c, g.nx) : G;
+    }
+};
+
+// strain-adjoint sweep: steps 0-T..2-T of fwi/adjoint_jax_kernel.py:201-323;
+// produces the final U_a, V_a of the step and the source-wavelet adjoint
+template <typename T, int FD>
+__global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
+{
+    const Geo<T> &g = a.g;
+    const int s = blockIdx.z;
+    const ShotDev sh = a.shots[s];
+    if (it >= sh.nt) return;
+    const int tz = blockIdx.x / a.tiles_x, tx = blockIdx.x - tz * a.tiles_x;
+    const int r = tz * FWD_TILE_Z + threadIdx.y, c = tx * FWD_TILE_X + threadIdx.x;
+    if (r >= g.nz || c >= g.nx) return;
+    constexpr int NK = FDC<FD>::NK;
+    const int nz = g.nz, nx = g.nx;
+    T *st = a.state + s * a.sstride + g.origin;
+    T *Ua = st, *Va = st + g.fstride;
+    const K4Src<T> S{g, st + 4 * g.fstride, st + 2 * g.fstride, st + 3 * g.fstride,
+                     a.psix_o + s * a.psix_stride, a.psiz_o + s * a.psiz_stride,
+                     a.psix_len, a.psiz_len, a.profx, a.profz, a.Lr0};
+    const i64 q = (i64)r * g.ld + c;
+    T ua = Ua[q], va = Va[q];
+#pragma unroll
+    for (int k = 1; k <= NK; ++k) {
+        const T ck = (T)fdc<FD>(k);
+        // (D+z)^T of Dz_U_a, columns [1, nx-1]; term range [k-1, nz-1-k]
+        if (c >= 1) {
+            if (r - k >= k - 1 && r - k <= nz - 1 - k) ua += ck * S.DzUa(r - k, c);
+            if (r + k - 1 >= k - 1 && r + k - 1 <= nz - 1 - k) ua += -ck * S.DzUa(r + k - 1, c);
+        }
+        // (D+x)^T of Dx_V_a, rows [0, nz-2]
+        if (r <= nz - 2) {
+            if (c - k >= k - 1 && c - k <= nx - 1 - k) va += ck * S.DxVa(r, c - k);
+            if (c + k - 1 >= k - 1 && c + k - 1 <= nx - 1 - k) va += -ck * S.DxVa(r, c + k - 1);
+        }
+    }
+#pragma unroll
+    for (int k = 1; k <= NK; ++k) {
+        const T ck = (T)fdc<FD>(k);
+        // (D-z)^T of Dz_V_a, columns [0, nx-2]; term 1 rows [1, nz-1], k: [k, nz-k]
+        if (c <= nx - 2) {
+            const int i1 = r - k + 1, i2 = r + k;
+            if (k == 1 ? (i1 >= 1 && i1 <= nz - 1) : (i1 >= k && i1 <= nz - k)) va += ck * S.DzVa(i1, c);
+            if (k == 1 ? (i2 >= 1 && i2 <= nz - 1) : (i2 >= k && i2 <= nz - k)) va += -ck * S.DzVa(i2, c);
+        }
+    }
+#pragma unroll
+    for (int k = 1; k <= NK; ++k) {
+        const T ck = (T)fdc<FD>(k);
+        // (D-x)^T of Dx_U_a, all rows; term 1 cols [1, nx-1], k: [k, nx-k]
+        const int i1 = c - k + 1, i2 = c + k;
+        if (k == 1 ? (i1 >= 1 && i1 <= nx - 1) : (i1 >= k && i1 <= nx - k)) ua += ck * S.DxUa(r, i1);
+        if (k == 1 ? (i2 >= 1 && i2 <= nx - 1) : (i2 >= k && i2 <= nx - k)) ua += -ck * S.DxUa(r, i2);
+    }
+    Ua[q] = ua;
+    Va[q] = va;
+    // C-PML memories of this cell
+    const int ix = xstrip(g, r, c), iz = zstrip(g, r, c);
+    if (iz >= 0) {
+        T *pz = a.psiz_n + s * a.psiz_stride + iz;
+        pz[0] = pmlT_psi(S.GzU(r, c), S.pzo[iz], a.profz, r, nz);
+        pz[a.psiz_len] = pmlT_psi(S.GzV(r, c), S.pzo[a.psiz_len + iz], a.profz, r, nz);
+    }
+    if (ix >= 0) {
+        T *px = a.psix_n + s * a.psix_stride + ix;
+        px[0] = pmlT_psi(S.GxV(r, c), S.pxo[ix], a.profx, c, nx);
+        px[a.psix_len] = pmlT_psi(S.GxU(r, c), S.pxo[a.psix_len + ix], a.profx, c, nx);
+    }
+    if (r == sh.sz && c == sh.sx && a.src_adj) {
+        T sa;
+        if (sh.type == 1) {
+            const T lm = (a.L2M[g.origin + q] - T(2) * a.M[g.origin + q]) + a.M[g.origin + q];
+            sa = (g.dt / (T(2) * lm * g.dx * g.dz)) * (S.exa[q] + S.eza[q]);
+        } else {
+            sa = g.fsrc * a.B[g.origin + q] * va;
+        }
+        a.src_adj[(i64)s * a.nt_max + it] = sa;
+    }
+}
+
+// adjoints of the velocity-update derivatives (fwi/adjoint_jax_kernel.py:325-399)
+template <typename T>
+struct K5Src {
+    const Geo<T> &g;
+    const T *Ua, *Va, *B;
+    const T *pxo, *pzo;   // x[2]=Dx_M_es x[3]=Dx_(B U);  z[2]=Dz_(B V) z[3]=Dz_M_es
+    i64 pxn, pzn;
+    const T *profx, *profz;
+    __device__ __forceinline__ bool inR(int r, int c) const { return r <= g.nz - 2 && c >= 1 && c <= g.nx - 2; }
+    __device__ __forceinline__ T GxBU(int r, int c) const
+    {
+        const i64 p = (i64)r * g.ld + c;
+        return inR(r, c) ? g.dtdx * B[p] * Ua[p] : T(0);
+    }
+    __device__ __forceinline__ T GzMes(int r, int c) const
+    {
+        const i64 p = (i64)r * g.ld + c;
+        return inR(r, c) ? g.dtdz * B[p] * Ua[p] : T(0);
+    }
+    __device__ __forceinline__ T GxMes(int r, int c) const
+    {
+        const i64 p = (i64)r * g.ld + c;
+        return inR(r, c) ? g.dtdx * B[p] * Va[p] : T(0);
+    }
+    __device__ __forceinline__ T GzBV(int r, int c) const
+    {
+        const i64 p = (i64)r * g.ld + c;
+        return inR(r, c) ? g.dtdz * B[p] * Va[p] : T(0);
+    }
+    __device__ __forceinline__ T DxBUa(int r, int c) const
+    {
+        const T G = GxBU(r, c);
+        const int ix = xstrip(g, r, c);
+        return ix >= 0 ? pmlT_val(G, pxo[3 * pxn + ix], profx, c, g.nx) : G;
+    }
+    __device__ __forceinline__ T DxMesa(int r, int c) const
+    {
+        const T G = GxMes(r, c);
+        const int ix = xstrip(g, r, c);
+        return ix >= 0 ? pmlT_val(G, pxo[2 * pxn + ix], profx, c, g.nx) : G;
+    }
+    __device__ __forceinline__ T DzBVa(int r, int c) const
+    {
+        const T G = GzBV(r, c);
+        const int iz = zstrip(g, r, c);
+        return iz >= 0 ? pmlT_val(G, pzo[2 * pzn + iz], profz, r, g.nz) : G;
+    }
+    __device__ __forceinline__ T DzMesa(int r, int c) const
+    {
+        const T G = GzMes(r, c);
+        const int iz = zstrip(g, r, c);
+        return iz >= 0 ? pmlT_val(G, pzo[3 * pzn + iz], profz, r, g.nz) : G;
+    }
+};
+
+// velocity-adjoint sweep with the imaging condition fused: steps 2-T (V),
+// 1-T (U) and the constitutive transposes (fwi/adjoint_jax_kernel.py:325-464)
+template <typename T, int FD>
+__global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
+{
+    const Geo<T> &g = a.g;
+    const int s = blockIdx.z;
+    const ShotDev sh = a.shots[s];
+    if (it >= sh.nt) return;
+    const int tz = blockIdx.x / a.tiles_x, tx = blockIdx.x - tz * a.tiles_x;
+    const int r = tz * FWD_TILE_Z + threadIdx.y, c = tx * FWD_TILE_X + threadIdx.x;
+    if (r >= g.nz || c >= g.nx) return;
+    constexpr int NK = FDC<FD>::NK;
+    const int nz = g.nz, nx = g.nx;
+    T *st = a.state + s * a.sstride + g.origin;
+    const T *Ua = st, *Va = st + g.fstride;
+    T *exa = st + 2 * g.fstride, *eza = st + 3 * g.fstride, *esa = st + 4 * g.fstride;
+    const K5Src<T> S{g, Ua, Va, a.B + g.origin, a.psix_o + s * a.psix_stride, a.psiz_o + s * a.psiz_stride,
+                     a.psix_len, a.psiz_len, a.profx, a.profz};
+    const i64 q = (i64)r * g.ld + c;
+    T Ax = 0, Az = 0, Mes = 0;
+#pragma unroll
+    for (int k = 1; k <= NK; ++k) {
+        const T ck = (T)fdc<FD>(k);
+        // (D+x)^T of Dx_(B U)_a (= L2M_ex_a = L_ez_a), rows [0, nz-2]
+        if (r <= nz - 2) {
+            if (c - k >= k - 1 && c - k <= nx - 1 - k) Ax += ck * S.DxBUa(r, c - k);
+            if (c + k - 1 >= k - 1 && c + k - 1 <= nx - 1 - k) Ax += -ck * S.DxBUa(r, c + k - 1);
+        }
+        // (D+z)^T of Dz_(B V)_a (= L2M_ez_a = L_ex_a), columns [1, nx-2]
+        if (c >= 1 && c <= nx - 2) {
+            if (r - k >= k - 1 && r - k <= nz - 1 - k) Az += ck * S.DzBVa(r - k, c);
+            if (r + k - 1 >= k - 1 && r + k - 1 <= nz - 1 - k) Az += -ck * S.DzBVa(r + k - 1, c);
+        }
+    }
+#pragma unroll
+    for (int k = 1; k <= NK; ++k) {
+        const T ck = (T)fdc<FD>(k);
+        // (D-x)^T of Dx_M_es_a, rows [0, nz-2]; term 1 cols [1, nx-2], k: [k, nx-k]
+        if (r <= nz - 2) {
+            const int i1 = c - k + 1, i2 = c + k;
+            if (k == 1 ? (i1 >= 1 && i1 <= nx - 2) : (i1 >= k && i1 <= nx - k)) Mes += ck * S.DxMesa(r, i1);
+            if (k == 1 ? (i2 >= 1 && i2 <= nx - 2) : (i2 >= k && i2 <= nx - k)) Mes += -ck * S.DxMesa(r, i2);
+        }
+    }
+#pragma unroll
+    for (int k = 1; k <= NK; ++k) {
+        const T ck = (T)fdc<FD>(k);
+        // (D-z)^T of Dz_M_es_a, columns [1, nx-2]; term 1 rows [1, nz-2], k: [k, nz-k]
+        if (c >= 1 && c <= nx - 2) {
+            const int i1 = r - k + 1, i2 = r + k;
+            if (k == 1 ? (i1 >= 1 && i1 <= nz - 2) : (i1 >= k && i1 <= nz - k)) Mes += ck * S.DzMesa(i1, c);
+            if (k == 1 ? (i2 >= 1 && i2 <= nz - 2) : (i2 >= k && i2 <= nz - k)) Mes += -ck * S.DzMesa(i2, c);
+        }
+    }
+    if (g.fs && r == 0 && c >= 1 && c <= nx - 2) Mes += T(2 * fdc<FD>(1)) * S.DzMesa(0, c);
+
+    const T l2m = a.L2M[g.origin + q], mu = a.M[g.origin + q];
+    const T lam = l2m - T(2) * mu;
+    const T exa0 = exa[q], eza0 = eza[q];
+    exa[q] = exa0 + l2m * Ax + lam * Az;
+    eza[q] = eza0 + lam * Ax + l2m * Az;
+    esa[q] = esa[q] + T(2) * mu * Mes;
+
+    const int ix = xstrip(g, r, c), iz = zstrip(g, r, c);
+    if (ix >= 0) {
+        T *px = a.psix_n + s * a.psix_stride + ix;
+        px[2 * a.psix_len] = pmlT_psi(S.GxMes(r, c), S.pxo[2 * a.psix_len + ix], a.profx, c, nx);
+        px[3 * a.psix_len] = pmlT_psi(S.GxBU(r, c), S.pxo[3 * a.psix_len + ix], a.profx, c, nx);
+    }
+    if (iz >= 0) {
+        T *pz = a.psiz_n + s * a.psiz_stride + iz;
+        pz[2 * a.psiz_len] = pmlT_psi(S.GzBV(r, c), S.pzo[2 * a.psiz_len + iz], a.profz, r, nz);
+        pz[3 * a.psiz_len] = pmlT_psi(S.GzMes(r, c), S.pzo[3 * a.psiz_len + iz], a.profz, r, nz);
+    }
+
+    if (a.snap) {
+        const T vals[5] = {Ua[q], Va[q], exa[q], eza[q], esa[q]};
+        int k = 0;
+        for (int f = 0; f < 5; ++f)
+            if (a.snap_mask & (1 << f)) {
+                a.snap[(((i64)s * a.nsnap + k) * a.nt_max + it) * nz * nx + (i64)r * nx + c] = vals[f];
+                ++k;
+            }
+    }
+
+    // imaging on the physical cells
+    const int rp = r - g.z0, cp = c - g.pad;
+    if (a.imaging == 0 || rp < 0 || rp >= g.nzd || cp < 0 || cp >= g.nxd) return;
+    const i64 ip = (i64)rp * g.nxd + cp;
+    T *acc = a.gacc + (i64)s * a.nacc * g.np + ip;
+    const i64 hs = (i64)s * a.nt_max;
+    if (a.imaging == 2) {
+        // exact discrete gradient: model dependence of step it, paired with
+        // its carry-in strains (history frame it-1)
+        T gl2m = 0, gl = 0, gm = 0, gb = 0;
+        if (it > 0) {
+            const T *h = a.hist + (hs + it - 1) * 3 * g.np + ip;
+            const T exm = h[0], ezm = h[g.np], esm = h[2 * g.np];
+            gl2m = exm * Ax + ezm * Az;
+            gl = ezm * Ax + exm * Az;
+            gm = T(2) * esm * Mes;
+        }
+        if (g.fs && r == 0 && c <= nx - 2) {
+            // sigma_zz = 0 row: Dz_V = L_ratio Dx_U dz/dx; row 0 has no z-PML
+            const T tt = g.dtdz * eza0 * a.dxu0[(hs + it) * nx + c] * g.dz / g.dx;
+            gl += -tt / l2m;
+            gl2m += tt * lam / (l2m * l2m);
+        }
+        if (sh.type == 1 && r == sh.sz && c == sh.sx) {
+            const T lm = lam + mu;
+            const T stm = a.wav[hs + it] * g.dt / (T(2) * lm * g.dx * g.dz);
+            const T Qm = -stm / lm * (exa0 + eza0);
+            gl += Qm;
+            gm += Qm;
+        }
+        if (a.gparams == 3) {
+            const T *h2 = a.hist2 + (hs + it) * 2 * g.np + ip;
+            gb = Ua[q] * h2[0] + Va[q] * h2[g.np];
+            acc[0] += gl2m;
+            acc[g.np] += gl;
+            acc[2 * g.np] += gm;
+            acc[3 * g.np] += gb;
+        } else {
+            acc[0] += gm - T(2) * gl;
+            if (a.gparams == 2) acc[g.np] += gl2m + gl;
+        }
+    } else {
+        // Zhou imaging (fwi/imaging.py:125-152): forward frame it with the
+        // adjoint carry-out of step it; centred periodic differences of B v_a
+        const T *h = a.hist + (hs + it) * 3 * g.np + ip;
+        const T ex = h[0], ez = h[g.np], es = h[2 * g.np];
+        const T *Bi = a.Bimg + g.origin;
+        const int cpl = c + 1 == nx ? 0 : c + 1, cmi = c == 0 ? nx - 1 : c - 1;
+        const int rpl = r + 1 == nz ? 0 : r + 1, rmi = r == 0 ? nz - 1 : r - 1;
+        const i64 qxp = (i64)r * g.ld + cpl, qxm = (i64)r * g.ld + cmi;
+        const i64 qzp = (i64)rpl * g.ld + c, qzm = (i64)rmi * g.ld + c;
+        const T i2dx = T(2) * g.dx, i2dz = T(2) * g.dz;
+        const T dxBU = (Bi[qxp] * Ua[qxp] - Bi[qxm] * Ua[qxm]) / i2dx;
+        const T dzBV = (Bi[qzp] * Va[qzp] - Bi[qzm] * Va[qzm]) / i2dz;
+        const T dzBU = (Bi[qzp] * Ua[qzp] - Bi[qzm] * Ua[qzm]) / i2dz;
+        const T dxBV = (Bi[qxp] * Va[qxp] - Bi[qxm] * Va[qxm]) / i2dx;
+        const T glam = (ex + ez) * (dxBU + dzBV);
+        const T gmu = T(2) * ex * dxBU + T(2) * ez * dzBV + es * (dzBU + dxBV);
+        if (a.gparams == 3) {
+            // density term: U_a(t) pairs with v(t-1) - v(t+1) (summation by
+            // parts of sum_{t=1}^{nt-2} v(t) (U_a(t+1) - U_a(t-1)) / 2dt)
+            T grl = 0;
+            const T i2dt = T(2) * g.dt;
+            if (it >= 2 && it <= sh.nt - 1) {
+                const T *h2 = a.hist2 + (hs + it - 1) * 2 * g.np + ip;
+                grl += Ua[q] * h2[0] / i2dt + Va[q] * h2[g.np] / i2dt;
+            }
+            if (it <= sh.nt - 3) {
+                const T *h2 = a.hist2 + (hs + it + 1) * 2 * g.np + ip;
+                grl -= Ua[q] * h2[0] / i2dt + Va[q] * h2[g.np] / i2dt;
+            }
+            acc[0] += glam;
+            acc[g.np] += gmu;
+            acc[2 * g.np] += grl;
+        } else {
+            acc[0] += gmu - T(2) * glam;
+            if (a.gparams == 2) acc[g.np] += glam;
+        }
+    }
+}
+
+// sum of per-shot accumulators and chain rule to (Vs, Vp, rho)
+template <typename T, typename C>
+__global__ void k_finalize(int n_shots, i64 np, int nacc, int imaging, int gparams, T dt,
+                           const T *__restrict__ gacc, const C *__restrict__ vs, const C *__restrict__ vp,
+                           const C *__restrict__ rho, T *out)
+{
+    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    T s[4] = {0, 0, 0, 0};
+    for (int sh = 0; sh < n_shots; ++sh)
+        for (int k = 0; k < nacc; ++k) s[k] += gacc[((i64)sh * nacc + k) * np + i];
+    const C v_s = vs[i], v_p = vp[i], d = rho[i];
+    T gvs = 0, gvp = 0, grho = 0;
+    if (imaging == 1) {
+        for (int k = 0; k < 4; ++k) s[k] = -dt * s[k];
+        if (gparams == 3) {
+            gvs = T(C(2) * d * v_s) * (s[1] - T(2) * s[0]);
+            gvp = T(C(2) * d * v_p) * s[0];
+            grho = T(v_p * v_p - C(2) * v_s * v_s) * s[0] + T(v_s * v_s) * s[1] + s[2];
+        } else {
+            gvs = T(C(2) * d * v_s) * s[0];
+            if (gparams == 2) gvp = T(C(2) * d * v_p) * s[1];
+        }
+    } else {
+        if (gparams == 3) {
+            const T tvs = T(v_s), tvp = T(v_p), td = T(d);
+            gvs = T(2) * td * tvs * (s[2] - T(2) * s[1]);
+            gvp = T(2) * td * tvp * (s[0] + s[1]);
+            grho = tvs * tvs * s[2] + tvp * tvp * s[0] + (tvp * tvp - T(2) * tvs * tvs) * s[1] - s[3] / (td * td);
+        } else {
+            gvs = T(2) * T(d) * T(v_s) * s[0];
+            if (gparams == 2) gvp = T(2) * T(d) * T(v_p) * s[1];
+        }
+    }
+    out[i] = gvs;
+    out[np + i] = gvp;
+    out[2 * np + i] = grho;
+}
+
+// ---------------------------------------------------------------------------
+// residual prep: taper, DF-I biquad cascade, misfit and its transpose
+// (fwi/gradient.py:139-156, fwi/filters.py:14-42); one block per shot
+// ---------------------------------------------------------------------------
+#define MAX_SOS 8
+template <typename T>
+struct ResArgs {
+    const ShotDev *shots;
+    const T *pred;        // [shot][4][nt_max][nrec_max]
+    const T *obs;         // [shot][nt_max][nrec_max]
+    T *rec_adj;           // [shot][4][nt_max][nrec_max]
+    T *loss;              // [shot]
+    int nt_max, nrec_max, comp, n_sos, n_taper;
+    T coef[MAX_SOS][5];   // b0 b1 b2 a1 a2 (normalised by a0)
+    T w;                  // scale / n_shots
+};
+
+template <typename T>
+__device__ __forceinline__ T taper_w(int t, int nt, int n_taper)
+{
+    const int n = n_taper < nt ? n_taper : nt;
+    if (n <= 0 || t < nt - n) return T(1);
+    const int k = t - (nt - n) + 1;   // 1..n
+    return (T)((cos((double)k / n * M_PI) + 1.0) / 2.0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_residual(ResArgs<T> a)
+{
+    const int s = blockIdx.x;
+    const ShotDev sh = a.shots[s];
+    const int nt = sh.nt;
+    const i64 chs = (i64)a.nt_max * a.nrec_max;
+    const T *pred = a.pred + ((i64)s * 4 + a.comp) * chs;
+    const T *obs = a.obs + (i64)s * chs;
+    T *adj = a.rec_adj + (i64)s * 4 * chs;
+    __shared__ double red[256];
+    double part = 0;
+    for (int j = threadIdx.x; j < a.nrec_max; j += blockDim.x) {
+        for (int ch = 0; ch < 4; ++ch)
+            if (ch != a.comp)
+                for (int t = 0; t < a.nt_max; ++t) adj[ch * chs + (i64)t * a.nrec_max + j] = 0;
+        T *rr = adj + a.comp * chs + j;
+        if (j >= sh.nrec) {
+            for (int t = 0; t < a.nt_max; ++t) rr[(i64)t * a.nrec_max] = 0;
+            continue;
+        }
+        // forward: r = F T pred - F T obs, stored in place
+        T xs[MAX_SOS][2][2], ys[MAX_SOS][2][2];   // [section][pred/obs][lag]
+        for (int k = 0; k < a.n_sos; ++k)
+            for (int u = 0; u < 2; ++u) xs[k][u][0] = xs[k][u][1] = ys[k][u][0] = ys[k][u][1] = 0;
+        T ssq = 0;
+        for (int t = 0; t < nt; ++t) {
+            const T tw = taper_w<T>(t, nt, a.n_taper);
+            T v[2] = {pred[(i64)t * a.nrec_max + j], obs[(i64)t * a.nrec_max + j]};
+            for (int u = 0; u < 2; ++u) {
+                if (a.n_taper > 0) v[u] = v[u] * tw;
+                for (int k = 0; k < a.n_sos; ++k) {
+                    const T *cf = a.coef[k];
+                    const T y = cf[0] * v[u] + cf[1] * xs[k][u][0] + cf[2] * xs[k][u][1] - cf[3] * ys[k][u][0] - cf[4] * ys[k][u][1];
+                    xs[k][u][1] = xs[k][u][0];
+                    xs[k][u][0] = v[u];
+                    ys[k][u][1] = ys[k][u][0];
+                    ys[k][u][0] = y;
+                    v[u] = y;
+                }
+            }
+            const T res = v[0] - v[1];
+            rr[(i64)t * a.nrec_max] = res;
+            ssq += res * res;
+        }
+        for (int t = nt; t < a.nt_max; ++t) rr[(i64)t * a.nrec_max] = 0;
+        part += (double)ssq;
+        // transpose: cot = 2 r w / N, then F^T (reverse-time recursion), T^T
+        const T g0 = T(2) * (a.w / (T)((double)nt * sh.nrec));
+        T cx1[MAX_SOS], cx2[MAX_SOS], cy1[MAX_SOS], cy2[MAX_SOS];
+        for (int k = 0; k < a.n_sos; ++k) cx1[k] = cx2[k] = cy1[k] = cy2[k] = 0;
+        for (int t = nt - 1; t >= 0; --t) {
+            T gcur = rr[(i64)t * a.nrec_max] * g0;
+            for (int k = a.n_sos - 1; k >= 0; --k) {
+                const T *cf = a.coef[k];
+                const T gy = gcur + cy1[k];
+                const T gx = cx1[k] + cf[0] * gy;
+                cx1[k] = cx2[k] + cf[1] * gy;
+                cx2[k] = cf[2] * gy;
+                cy1[k] = cy2[k] - cf[3] * gy;
+                cy2[k] = -cf[4] * gy;
+                gcur = gx;
+            }
+            if (a.n_taper > 0) gcur = gcur * taper_w<T>(t, nt, a.n_taper);
+            rr[(i64)t * a.nrec_max] = gcur;
+        }
+    }
+    red[threadIdx.x] = part;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.loss[s] = (T)((double)a.w * (red[0] / ((double)nt * sh.nrec)));
+}
+
+}  // namespace stfwi

@@ -1,0 +1,580 @@
+// Host side of libstfwi.so: plan lifecycle, launches and the C-ABI declared
+// in include/stfwi.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdarg>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/stfwi.h"
+#include "fwi_kernels.h"
+
+using namespace stfwi;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+static int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(x)                                                                          \
+    do {                                                                                    \
+        hipError_t e_ = (x);                                                                \
+        if (e_ != hipSuccess)                                                               \
+            return fail(STFWI_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(e_));         \
+    } while (0)
+
+struct stfwi_plan {
+    stfwi_config cfg;
+    int esz = 4;
+    int nz = 0, nx = 0, ld = 0, z0 = 0, zrows = 0, nacc = 1;
+    i64 fstride = 0, origin = 0, np = 0;
+    i64 psix_len = 0, psiz_len = 0;
+    // device buffers
+    void *mod = nullptr;      // L2M, M, B, Bimg fields (4 x fstride) followed by Lr0 (nx)
+    void *profx = nullptr, *profz = nullptr;
+    double *vpmax = nullptr;
+    void *state = nullptr;    // n_shots_max x 5 x fstride
+    void *psix_f = nullptr, *psiz_f = nullptr;          // forward: n_shots x 5 x len
+    void *psix_a[2] = {nullptr, nullptr}, *psiz_a[2] = {nullptr, nullptr};  // adjoint: n_shots x 4 x len
+    void *hist = nullptr, *hist2 = nullptr, *dxu0 = nullptr;
+    void *gacc = nullptr;
+    int hist2_mode = 0;
+    ShotDev *shots_d = nullptr;
+    i64 *rec_cell_d = nullptr, *uc_cell_d = nullptr;
+    int *uc_ptr_d = nullptr, *uc_rec_d = nullptr;
+    i64 bytes = 0;
+    // shots
+    int n_shots = 0, nt_run = 0, max_nuc = 0, max_nrec = 0;
+    const void *wav = nullptr;
+    bool have_model = false, fwd_done = false, adj_done = false;
+    std::vector<void *> allocs;
+};
+
+static int dalloc(stfwi_plan *p, void **ptr, size_t bytes)
+{
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        return fail(STFWI_ERR_OOM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    p->allocs.push_back(*ptr);
+    p->bytes += (i64)bytes;
+    return 0;
+}
+
+template <typename T>
+static void fill(void *ptr, i64 n, hipStream_t st)
+{
+    if (n <= 0) return;
+    int blocks = (int)std::min<i64>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_fill<T>, dim3(blocks), dim3(256), 0, st, (T *)ptr, n, T(0));
+}
+
+template <typename T>
+static Geo<T> make_geo(const stfwi_plan *p)
+{
+    const stfwi_config &c = p->cfg;
+    Geo<T> g;
+    g.nz = p->nz; g.nx = p->nx; g.ld = p->ld;
+    g.z0 = p->z0; g.pad = c.pad; g.nzd = c.nz_dom; g.nxd = c.nx_dom;
+    g.fs = c.free_surface; g.zrows = p->zrows;
+    g.fstride = p->fstride; g.origin = p->origin; g.np = p->np;
+    g.dt = (T)c.dt; g.dx = (T)c.dx; g.dz = (T)c.dz;
+    g.dtdx = (T)(c.dt / c.dx); g.dtdz = (T)(c.dt / c.dz);
+    g.hdtdx = (T)(0.5 * c.dt / c.dx); g.hdtdz = (T)(0.5 * c.dt / c.dz);
+    g.dzdx = (T)(c.dz / c.dx);
+    g.fsrc = (T)(c.dt / (c.dx * c.dz));
+    return g;
+}
+
+extern "C" int stfwi_abi_version(void) { return STFWI_ABI_VERSION; }
+extern "C" const char *stfwi_last_error(void) { return g_err.c_str(); }
+
+extern "C" long long stfwi_plan_bytes(const stfwi_plan *plan) { return plan ? plan->bytes : 0; }
+
+extern "C" int stfwi_plan_destroy(stfwi_plan *plan)
+{
+    if (!plan) return 0;
+    for (void *a : plan->allocs) (void)hipFree(a);
+    delete plan;
+    return 0;
+}
+
+extern "C" int stfwi_plan_create(const stfwi_config *cfg, stfwi_plan **out)
+{
+    if (!cfg || !out) return fail(STFWI_ERR_INVALID, "null argument");
+    *out = nullptr;
+    const stfwi_config &c = *cfg;
+    if (c.fd_order != 2 && c.fd_order != 4 && c.fd_order != 8)
+        return fail(STFWI_ERR_UNSUPPORTED, "fd_order=%d not supported (use 2, 4, or 8)", c.fd_order);
+    if (c.dtype != STFWI_F32 && c.dtype != STFWI_F64) return fail(STFWI_ERR_INVALID, "dtype=%d", c.dtype);
+    if (c.nz_dom < 1 || c.nx_dom < 1 || c.pad < 1 || c.n_shots_max < 1 || c.nt_max < 1 || c.nrec_max < 0)
+        return fail(STFWI_ERR_INVALID, "bad sizes");
+    if (c.imaging < 0 || c.imaging > 2 || c.grad_params < 1 || c.grad_params > 3)
+        return fail(STFWI_ERR_INVALID, "imaging=%d grad_params=%d", c.imaging, c.grad_params);
+    if (c.dx <= 0 || c.dz <= 0 || c.dt <= 0) return fail(STFWI_ERR_INVALID, "dx, dz, dt must be positive");
+    stfwi_plan *p = new stfwi_plan();
+    p->cfg = c;
+    p->esz = c.dtype == STFWI_F64 ? 8 : 4;
+    p->nz = c.nz_dom + (c.free_surface ? c.pad : 2 * c.pad);
+    p->nx = c.nx_dom + 2 * c.pad;
+    if (p->nz < 10 || p->nx < 10) {
+        delete p;
+        return fail(STFWI_ERR_INVALID, "padded grid %dx%d too small", p->nz, p->nx);
+    }
+    p->z0 = c.free_surface ? 0 : c.pad;
+    p->zrows = c.free_surface ? c.pad : 2 * c.pad;
+    p->ld = ((p->nx + 8 + 63) / 64) * 64;
+    p->fstride = (i64)(p->nz + 8) * p->ld;
+    p->origin = (i64)4 * p->ld;
+    p->np = (i64)c.nz_dom * c.nx_dom;
+    p->psix_len = (i64)p->nz * 2 * c.pad;
+    p->psiz_len = (i64)p->zrows * p->nx;
+    p->nacc = c.grad_params == 3 ? (c.imaging == 1 ? 3 : 4) : c.grad_params;
+    const size_t es = p->esz;
+    const i64 S = c.n_shots_max;
+    int rc = 0;
+    rc |= dalloc(p, &p->mod, (4 * p->fstride + p->nx) * es);
+    rc |= dalloc(p, &p->profx, 4 * p->nx * es);
+    rc |= dalloc(p, &p->profz, 4 * p->nz * es);
+    rc |= dalloc(p, (void **)&p->vpmax, sizeof(double));
+    rc |= dalloc(p, &p->state, S * 5 * p->fstride * es);
+    rc |= dalloc(p, &p->psix_f, S * 5 * p->psix_len * es);
+    rc |= dalloc(p, &p->psiz_f, S * 5 * p->psiz_len * es);
+    for (int k = 0; k < 2; ++k) {
+        rc |= dalloc(p, &p->psix_a[k], S * 4 * p->psix_len * es);
+        rc |= dalloc(p, &p->psiz_a[k], S * 4 * p->psiz_len * es);
+    }
+    if (c.store_history && c.imaging) {
+        rc |= dalloc(p, &p->hist, (size_t)S * c.nt_max * 3 * p->np * es);
+        if (c.grad_params == 3) {
+            p->hist2_mode = c.imaging == 2 ? 1 : 2;
+            rc |= dalloc(p, &p->hist2, (size_t)S * c.nt_max * 2 * p->np * es);
+        }
+        if (c.imaging == 2 && c.free_surface) rc |= dalloc(p, &p->dxu0, (size_t)S * c.nt_max * p->nx * es);
+        rc |= dalloc(p, &p->gacc, (size_t)S * p->nacc * p->np * es);
+    }
+    rc |= dalloc(p, (void **)&p->shots_d, S * sizeof(ShotDev));
+    rc |= dalloc(p, (void **)&p->rec_cell_d, (S * std::max(c.nrec_max, 1)) * sizeof(i64));
+    rc |= dalloc(p, (void **)&p->uc_cell_d, (S * std::max(c.nrec_max, 1)) * sizeof(i64));
+    rc |= dalloc(p, (void **)&p->uc_ptr_d, (S * std::max(c.nrec_max, 1) + 1) * sizeof(int));
+    rc |= dalloc(p, (void **)&p->uc_rec_d, (S * std::max(c.nrec_max, 1)) * sizeof(int));
+    if (rc) {
+        std::string e = g_err;
+        stfwi_plan_destroy(p);
+        g_err = e;
+        return STFWI_ERR_OOM;
+    }
+    // margins of the moduli buffers must be zero
+    hipStream_t st = nullptr;
+    if (c.dtype == STFWI_F64) fill<double>(p->mod, 4 * p->fstride + p->nx, st);
+    else fill<float>(p->mod, 4 * p->fstride + p->nx, st);
+    HIP_TRY(hipStreamSynchronize(st));
+    *out = p;
+    return 0;
+}
+
+template <typename T>
+static int set_model_t(stfwi_plan *p, const void *vs, const void *vp, const void *rho, const void *b_img,
+                       hipStream_t st)
+{
+    Geo<T> g = make_geo<T>(p);
+    T *mod = (T *)p->mod;
+    T *L2M = mod, *M = mod + p->fstride, *B = mod + 2 * p->fstride, *Bimg = mod + 3 * p->fstride;
+    T *Lr0 = mod + 4 * p->fstride;
+    dim3 grid((p->nx + 127) / 128, p->nz);
+    hipLaunchKernelGGL(k_model<T>, grid, dim3(128), 0, st, g, (const T *)vs, (const T *)vp, (const T *)rho,
+                       (const T *)b_img, L2M, M, B, Bimg, Lr0);
+    hipLaunchKernelGGL(k_vpmax<T>, dim3(1), dim3(256), 0, st, (const T *)vp, p->np, p->vpmax);
+    const int n = std::max(p->nx, p->nz);
+    hipLaunchKernelGGL(k_profiles<T>, dim3((n + 127) / 128), dim3(128), 0, st, p->nz, p->nx, p->cfg.pad,
+                       p->cfg.free_surface, p->cfg.nz_dom, p->cfg.dx, p->cfg.dt, p->cfg.fc, p->vpmax,
+                       (T *)p->profx, (T *)p->profz);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int stfwi_set_model(stfwi_plan *p, const void *vs, const void *vp, const void *rho, const void *b_img,
+                               void *stream)
+{
+    if (!p || !vs || !vp || !rho) return fail(STFWI_ERR_INVALID, "null argument");
+    int rc = p->cfg.dtype == STFWI_F64 ? set_model_t<double>(p, vs, vp, rho, b_img, (hipStream_t)stream)
+                                       : set_model_t<float>(p, vs, vp, rho, b_img, (hipStream_t)stream);
+    if (rc == 0) p->have_model = true;
+    p->fwd_done = false;
+    return rc;
+}
+
+extern "C" int stfwi_set_shots(stfwi_plan *p, const stfwi_shot *shots, int n_shots, const int *rec_x,
+                               const int *rec_z, const void *wavelets, void *stream)
+{
+    if (!p || !shots || !wavelets) return fail(STFWI_ERR_INVALID, "null argument");
+    const stfwi_config &c = p->cfg;
+    if (n_shots < 1 || n_shots > c.n_shots_max)
+        return fail(STFWI_ERR_INVALID, "n_shots=%d outside [1, %d]", n_shots, c.n_shots_max);
+    std::vector<ShotDev> sd(n_shots);
+    std::vector<i64> cells(n_shots * (size_t)std::max(c.nrec_max, 1));
+    std::vector<i64> ucell;
+    std::vector<int> uptr(1, 0), urec;
+    int nt_run = 0, max_nuc = 0, max_nrec = 0;
+    for (int s = 0; s < n_shots; ++s) {
+        const stfwi_shot &h = shots[s];
+        if (h.src_type != STFWI_SRC_FORCE && h.src_type != STFWI_SRC_MOMENT)
+            return fail(STFWI_ERR_UNSUPPORTED, "src_type=%d not supported (use force or moment)", h.src_type);
+        if (h.nt < 1 || h.nt > c.nt_max) return fail(STFWI_ERR_INVALID, "shot %d: nt=%d outside [1, %d]", s, h.nt, c.nt_max);
+        if (h.nrec < 1 || h.nrec > c.nrec_max)
+            return fail(STFWI_ERR_INVALID, "shot %d: nrec=%d outside [1, %d]", s, h.nrec, c.nrec_max);
+        if (h.src_x < 0 || h.src_x >= c.nx_dom || h.src_z < 0 || h.src_z >= c.nz_dom)
+            return fail(STFWI_ERR_INVALID, "shot %d: source (%d,%d) outside the model", s, h.src_z, h.src_x);
+        ShotDev &d = sd[s];
+        d.sx = h.src_x + c.pad;
+        d.sz = h.src_z + p->z0;
+        d.type = h.src_type;
+        d.nt = h.nt;
+        d.rec_off = s * c.nrec_max;
+        d.nrec = h.nrec;
+        d.uc_off = (int)ucell.size();
+        std::map<i64, std::vector<int>> byc;
+        for (int j = 0; j < h.nrec; ++j) {
+            const int rx = rec_x[h.rec_offset + j], rz = rec_z[h.rec_offset + j];
+            if (rx < 0 || rx >= c.nx_dom || rz < 0 || rz >= c.nz_dom)
+                return fail(STFWI_ERR_INVALID, "shot %d receiver %d: (%d,%d) outside the model", s, j, rz, rx);
+            const i64 cell = (i64)(rz + p->z0) * p->ld + rx + c.pad;
+            cells[(size_t)s * c.nrec_max + j] = cell;
+            byc[cell].push_back(j);
+        }
+        for (auto &kv : byc) {
+            ucell.push_back(kv.first);
+            for (int j : kv.second) urec.push_back(j);
+            uptr.push_back((int)urec.size());
+        }
+        d.nuc = (int)ucell.size() - d.uc_off;
+        nt_run = std::max(nt_run, h.nt);
+        max_nuc = std::max(max_nuc, d.nuc);
+        max_nrec = std::max(max_nrec, h.nrec);
+    }
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(hipMemcpyAsync(p->shots_d, sd.data(), sd.size() * sizeof(ShotDev), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(p->rec_cell_d, cells.data(), cells.size() * sizeof(i64), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(p->uc_cell_d, ucell.data(), ucell.size() * sizeof(i64), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(p->uc_ptr_d, uptr.data(), uptr.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(p->uc_rec_d, urec.data(), urec.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    // the host vectors die at return: wait for the copies
+    HIP_TRY(hipStreamSynchronize(st));
+    p->n_shots = n_shots;
+    p->nt_run = nt_run;
+    p->max_nuc = max_nuc;
+    p->max_nrec = max_nrec;
+    p->wav = wavelets;
+    p->fwd_done = false;
+    return 0;
+}
+
+static int popcount5(int m)
+{
+    int n = 0;
+    for (int f = 0; f < 5; ++f) n += (m >> f) & 1;
+    return n;
+}
+
+template <typename T, int FD>
+static int forward_t(stfwi_plan *p, void *rec, int rec_mask, void *illum, int illum_comp, void *snap, int snap_mask,
+                     hipStream_t st)
+{
+    const stfwi_config &c = p->cfg;
+    FwdArgs<T> a;
+    a.g = make_geo<T>(p);
+    a.shots = p->shots_d;
+    const T *mod = (const T *)p->mod;
+    a.L2M = mod; a.M = mod + p->fstride; a.B = mod + 2 * p->fstride; a.Lr0 = mod + 4 * p->fstride;
+    a.state = (T *)p->state; a.sstride = 5 * p->fstride;
+    a.psix = (T *)p->psix_f; a.psiz = (T *)p->psiz_f;
+    a.psix_n = p->psix_len; a.psiz_n = p->psiz_len;
+    a.psix_stride = 5 * p->psix_len; a.psiz_stride = 5 * p->psiz_len;
+    a.profx = (const T *)p->profx; a.profz = (const T *)p->profz;
+    a.wav = (const T *)p->wav; a.nt_max = c.nt_max;
+    a.rec = (T *)rec; a.nrec_max = c.nrec_max; a.rec_mask = rec_mask;
+    a.rec_cell = p->rec_cell_d;
+    a.hist = (T *)p->hist;
+    a.hist2 = (T *)p->hist2; a.hist2_mode = p->hist ? p->hist2_mode : 0;
+    a.dxu0 = (T *)p->dxu0;
+    a.illum = (T *)illum; a.illum_comp = illum_comp;
+    a.snap = (T *)snap; a.snap_mask = snap_mask; a.nsnap = popcount5(snap_mask);
+    a.tiles_x = (p->nx + FWD_TILE_X - 1) / FWD_TILE_X;
+    a.tiles_z = (p->nz + FWD_TILE_Z - 1) / FWD_TILE_Z;
+    a.ntiles = a.tiles_x * a.tiles_z;
+    const int S = p->n_shots;
+    fill<T>(p->state, (i64)S * 5 * p->fstride, st);
+    fill<T>(p->psix_f, (i64)S * 5 * p->psix_len, st);
+    fill<T>(p->psiz_f, (i64)S * 5 * p->psiz_len, st);
+    const int rblk = rec ? (p->max_nrec + 255) / 256 : 0;
+    dim3 blk(FWD_TILE_X, FWD_TILE_Z);
+    dim3 gv(a.ntiles + rblk, 1, S), gs(a.ntiles + rblk, 1, S);
+    for (int it = 0; it < p->nt_run; ++it) {
+        hipLaunchKernelGGL((k_fwd_vel<T, FD>), gv, blk, 0, st, a, it);
+        hipLaunchKernelGGL((k_fwd_strain<T, FD>), gs, blk, 0, st, a, it);
+    }
+    // exx/ezz receivers of the last step
+    if (rec) hipLaunchKernelGGL((k_fwd_vel<T, FD>), gv, blk, 0, st, a, p->nt_run);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int stfwi_forward(stfwi_plan *p, void *rec, int rec_mask, void *illum, int illum_comp, void *snap,
+                             int snap_mask, void *stream)
+{
+    if (!p) return fail(STFWI_ERR_INVALID, "null plan");
+    if (!p->have_model) return fail(STFWI_ERR_STATE, "stfwi_set_model not called");
+    if (!p->n_shots) return fail(STFWI_ERR_STATE, "stfwi_set_shots not called");
+    if (illum && (illum_comp < 0 || illum_comp > 3)) return fail(STFWI_ERR_INVALID, "illum_comp=%d", illum_comp);
+    hipStream_t st = (hipStream_t)stream;
+    int rc;
+    const int fd = p->cfg.fd_order;
+    if (p->cfg.dtype == STFWI_F64)
+        rc = fd == 8 ? forward_t<double, 8>(p, rec, rec_mask, illum, illum_comp, snap, snap_mask, st)
+           : fd == 4 ? forward_t<double, 4>(p, rec, rec_mask, illum, illum_comp, snap, snap_mask, st)
+                     : forward_t<double, 2>(p, rec, rec_mask, illum, illum_comp, snap, snap_mask, st);
+    else
+        rc = fd == 8 ? forward_t<float, 8>(p, rec, rec_mask, illum, illum_comp, snap, snap_mask, st)
+           : fd == 4 ? forward_t<float, 4>(p, rec, rec_mask, illum, illum_comp, snap, snap_mask, st)
+                     : forward_t<float, 2>(p, rec, rec_mask, illum, illum_comp, snap, snap_mask, st);
+    p->fwd_done = rc == 0 && p->hist != nullptr;
+    return rc;
+}
+
+template <typename T, int FD>
+static int adjoint_t(stfwi_plan *p, const void *rec_adj, void *src_adj, void *snap, int snap_mask, hipStream_t st)
+{
+    const stfwi_config &c = p->cfg;
+    AdjArgs<T> a;
+    a.g = make_geo<T>(p);
+    a.shots = p->shots_d;
+    const T *mod = (const T *)p->mod;
+    a.L2M = mod; a.M = mod + p->fstride; a.B = mod + 2 * p->fstride; a.Bimg = mod + 3 * p->fstride;
+    a.Lr0 = mod + 4 * p->fstride;
+    a.state = (T *)p->state; a.sstride = 5 * p->fstride;
+    a.psix_len = p->psix_len; a.psiz_len = p->psiz_len;
+    a.psix_stride = 4 * p->psix_len; a.psiz_stride = 4 * p->psiz_len;
+    a.profx = (const T *)p->profx; a.profz = (const T *)p->profz;
+    a.wav = (const T *)p->wav; a.nt_max = c.nt_max;
+    a.rec_adj = (const T *)rec_adj; a.nrec_max = c.nrec_max;
+    a.uc_cell = p->uc_cell_d; a.uc_ptr = p->uc_ptr_d; a.uc_rec = p->uc_rec_d;
+    a.src_adj = (T *)src_adj;
+    a.imaging = p->hist ? c.imaging : 0;
+    a.gparams = c.grad_params; a.nacc = p->nacc;
+    a.hist = (const T *)p->hist; a.hist2 = (const T *)p->hist2; a.dxu0 = (const T *)p->dxu0;
+    a.gacc = (T *)p->gacc;
+    a.snap = (T *)snap; a.snap_mask = snap_mask; a.nsnap = popcount5(snap_mask);
+    a.tiles_x = (p->nx + FWD_TILE_X - 1) / FWD_TILE_X;
+    a.tiles_z = (p->nz + FWD_TILE_Z - 1) / FWD_TILE_Z;
+    a.ntiles = a.tiles_x * a.tiles_z;
+    const int S = p->n_shots;
+    fill<T>(p->state, (i64)S * 5 * p->fstride, st);
+    for (int k = 0; k < 2; ++k) {
+        fill<T>(p->psix_a[k], (i64)S * 4 * p->psix_len, st);
+        fill<T>(p->psiz_a[k], (i64)S * 4 * p->psiz_len, st);
+    }
+    if (a.imaging) fill<T>(p->gacc, (i64)S * p->nacc * p->np, st);
+    dim3 blk(FWD_TILE_X, FWD_TILE_Z), grid(a.ntiles, 1, S);
+    dim3 gi((p->max_nuc + 63) / 64, 1, S);
+    for (int it = p->nt_run - 1; it >= 0; --it) {
+        const int cur = it & 1, prev = cur ^ 1;
+        a.psix_o = (const T *)p->psix_a[prev]; a.psiz_o = (const T *)p->psiz_a[prev];
+        a.psix_n = (T *)p->psix_a[cur]; a.psiz_n = (T *)p->psiz_a[cur];
+        hipLaunchKernelGGL(k_adj_inject<T>, gi, dim3(64), 0, st, a, it);
+        hipLaunchKernelGGL((k_adj_strain<T, FD>), grid, blk, 0, st, a, it);
+        hipLaunchKernelGGL((k_adj_vel<T, FD>), grid, blk, 0, st, a, it);
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int stfwi_adjoint(stfwi_plan *p, const void *rec_adj, void *src_adj, void *snap, int snap_mask,
+                             void *stream)
+{
+    if (!p || !rec_adj) return fail(STFWI_ERR_INVALID, "null argument");
+    if (!p->have_model || !p->n_shots) return fail(STFWI_ERR_STATE, "model or shots not set");
+    if (p->cfg.imaging && p->hist && !p->fwd_done)
+        return fail(STFWI_ERR_STATE, "imaging needs a stfwi_forward with the same shots first");
+    hipStream_t st = (hipStream_t)stream;
+    const int fd = p->cfg.fd_order;
+    int rc;
+    if (p->cfg.dtype == STFWI_F64)
+        rc = fd == 8 ? adjoint_t<double, 8>(p, rec_adj, src_adj, snap, snap_mask, st)
+           : fd == 4 ? adjoint_t<double, 4>(p, rec_adj, src_adj, snap, snap_mask, st)
+                     : adjoint_t<double, 2>(p, rec_adj, src_adj, snap, snap_mask, st);
+    else
+        rc = fd == 8 ? adjoint_t<float, 8>(p, rec_adj, src_adj, snap, snap_mask, st)
+           : fd == 4 ? adjoint_t<float, 4>(p, rec_adj, src_adj, snap, snap_mask, st)
+                     : adjoint_t<float, 2>(p, rec_adj, src_adj, snap, snap_mask, st);
+    p->adj_done = rc == 0 && p->gacc != nullptr && p->cfg.imaging != 0;
+    return rc;
+}
+
+template <typename T, typename C>
+static int finalize_t(stfwi_plan *p, const void *vs, const void *vp, const void *rho, void *out, hipStream_t st)
+{
+    const int blocks = (int)((p->np + 255) / 256);
+    hipLaunchKernelGGL((k_finalize<T, C>), dim3(blocks), dim3(256), 0, st, p->n_shots, p->np, p->nacc,
+                       p->cfg.imaging, p->cfg.grad_params, (T)p->cfg.dt, (const T *)p->gacc, (const C *)vs,
+                       (const C *)vp, (const C *)rho, (T *)out);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int stfwi_gradient_finalize(stfwi_plan *p, const void *vs, const void *vp, const void *rho,
+                                       int chain_dtype, void *g_out, void *stream)
+{
+    if (!p || !vs || !vp || !rho || !g_out) return fail(STFWI_ERR_INVALID, "null argument");
+    if (!p->adj_done) return fail(STFWI_ERR_STATE, "no imaging adjoint has run");
+    hipStream_t st = (hipStream_t)stream;
+    const bool t64 = p->cfg.dtype == STFWI_F64;
+    if (chain_dtype == STFWI_F64)
+        return t64 ? finalize_t<double, double>(p, vs, vp, rho, g_out, st) : finalize_t<float, double>(p, vs, vp, rho, g_out, st);
+    if (chain_dtype == STFWI_F32)
+        return t64 ? finalize_t<double, float>(p, vs, vp, rho, g_out, st) : finalize_t<float, float>(p, vs, vp, rho, g_out, st);
+    return fail(STFWI_ERR_INVALID, "chain_dtype=%d", chain_dtype);
+}
+
+template <typename T>
+static int residual_t(stfwi_plan *p, const void *pred, const void *obs, int comp, const double *sos, int n_sos,
+                      int n_taper, double scale, int n_norm, void *loss, void *rec_adj, hipStream_t st)
+{
+    ResArgs<T> a;
+    a.shots = p->shots_d;
+    a.pred = (const T *)pred; a.obs = (const T *)obs; a.rec_adj = (T *)rec_adj; a.loss = (T *)loss;
+    a.nt_max = p->cfg.nt_max; a.nrec_max = p->cfg.nrec_max; a.comp = comp;
+    a.n_sos = n_sos; a.n_taper = n_taper;
+    for (int k = 0; k < n_sos; ++k) {
+        const double *s = sos + 6 * k;
+        a.coef[k][0] = (T)(s[0] / s[3]); a.coef[k][1] = (T)(s[1] / s[3]); a.coef[k][2] = (T)(s[2] / s[3]);
+        a.coef[k][3] = (T)(s[4] / s[3]); a.coef[k][4] = (T)(s[5] / s[3]);
+    }
+    a.w = (T)(scale / n_norm);
+    hipLaunchKernelGGL(k_residual<T>, dim3(p->n_shots), dim3(256), 0, st, a);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int stfwi_residual(stfwi_plan *p, const void *pred, const void *obs, int component, const double *sos,
+                              int n_sos, int n_taper, double scale, int n_shots_norm, void *loss_out, void *rec_adj,
+                              void *stream)
+{
+    if (!p || !pred || !obs || !loss_out || !rec_adj) return fail(STFWI_ERR_INVALID, "null argument");
+    if (!p->n_shots) return fail(STFWI_ERR_STATE, "stfwi_set_shots not called");
+    if (component < 0 || component > 3) return fail(STFWI_ERR_INVALID, "component=%d", component);
+    if (n_sos < 0 || n_sos > MAX_SOS || (n_sos > 0 && !sos)) return fail(STFWI_ERR_INVALID, "n_sos=%d", n_sos);
+    if (n_shots_norm < 1) return fail(STFWI_ERR_INVALID, "n_shots_norm=%d", n_shots_norm);
+    for (int k = 0; k < n_sos; ++k)
+        if (sos[6 * k + 3] == 0.0) return fail(STFWI_ERR_INVALID, "sos section %d has a0 = 0", k);
+    hipStream_t st = (hipStream_t)stream;
+    return p->cfg.dtype == STFWI_F64
+               ? residual_t<double>(p, pred, obs, component, sos, n_sos, n_taper, scale, n_shots_norm, loss_out, rec_adj, st)
+               : residual_t<float>(p, pred, obs, component, sos, n_sos, n_taper, scale, n_shots_norm, loss_out, rec_adj, st);
+}
+
+// ---------------------------------------------------------------------------
+// diagnostics: time `reps` launches of one time-step kernel for all shots of
+// the plan at a mid-record step, bracketed by HIP events on `stream`
+// ---------------------------------------------------------------------------
+template <typename T, int FD>
+static int time_kernel_t(stfwi_plan *p, int which, int reps, hipStream_t st, double *ms)
+{
+    const stfwi_config &c = p->cfg;
+    const int it = p->nt_run / 2;
+    dim3 blk(FWD_TILE_X, FWD_TILE_Z);
+    const int tiles_x = (p->nx + FWD_TILE_X - 1) / FWD_TILE_X, tiles_z = (p->nz + FWD_TILE_Z - 1) / FWD_TILE_Z;
+    const int ntiles = tiles_x * tiles_z;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    if (which <= 1) {
+        FwdArgs<T> a;
+        memset(&a, 0, sizeof a);
+        a.g = make_geo<T>(p);
+        a.shots = p->shots_d;
+        const T *mod = (const T *)p->mod;
+        a.L2M = mod; a.M = mod + p->fstride; a.B = mod + 2 * p->fstride; a.Lr0 = mod + 4 * p->fstride;
+        a.state = (T *)p->state; a.sstride = 5 * p->fstride;
+        a.psix = (T *)p->psix_f; a.psiz = (T *)p->psiz_f;
+        a.psix_n = p->psix_len; a.psiz_n = p->psiz_len;
+        a.psix_stride = 5 * p->psix_len; a.psiz_stride = 5 * p->psiz_len;
+        a.profx = (const T *)p->profx; a.profz = (const T *)p->profz;
+        a.wav = (const T *)p->wav; a.nt_max = c.nt_max; a.nrec_max = c.nrec_max;
+        a.rec_cell = p->rec_cell_d;
+        a.hist = (T *)p->hist; a.hist2 = (T *)p->hist2; a.hist2_mode = p->hist ? p->hist2_mode : 0;
+        a.dxu0 = (T *)p->dxu0;
+        a.tiles_x = tiles_x; a.tiles_z = tiles_z; a.ntiles = ntiles;
+        dim3 grid(ntiles, 1, p->n_shots);
+        HIP_TRY(hipEventRecord(e0, st));
+        for (int r = 0; r < reps; ++r) {
+            if (which == 0) hipLaunchKernelGGL((k_fwd_vel<T, FD>), grid, blk, 0, st, a, it);
+            else hipLaunchKernelGGL((k_fwd_strain<T, FD>), grid, blk, 0, st, a, it);
+        }
+        HIP_TRY(hipEventRecord(e1, st));
+    } else {
+        AdjArgs<T> a;
+        memset(&a, 0, sizeof a);
+        a.g = make_geo<T>(p);
+        a.shots = p->shots_d;
+        const T *mod = (const T *)p->mod;
+        a.L2M = mod; a.M = mod + p->fstride; a.B = mod + 2 * p->fstride; a.Bimg = mod + 3 * p->fstride;
+        a.Lr0 = mod + 4 * p->fstride;
+        a.state = (T *)p->state; a.sstride = 5 * p->fstride;
+        a.psix_len = p->psix_len; a.psiz_len = p->psiz_len;
+        a.psix_stride = 4 * p->psix_len; a.psiz_stride = 4 * p->psiz_len;
+        a.psix_o = (const T *)p->psix_a[0]; a.psiz_o = (const T *)p->psiz_a[0];
+        a.psix_n = (T *)p->psix_a[1]; a.psiz_n = (T *)p->psiz_a[1];
+        a.profx = (const T *)p->profx; a.profz = (const T *)p->profz;
+        a.wav = (const T *)p->wav; a.nt_max = c.nt_max; a.nrec_max = c.nrec_max;
+        a.imaging = p->hist ? c.imaging : 0;
+        a.gparams = c.grad_params; a.nacc = p->nacc;
+        a.hist = (const T *)p->hist; a.hist2 = (const T *)p->hist2; a.dxu0 = (const T *)p->dxu0;
+        a.gacc = (T *)p->gacc;
+        a.tiles_x = tiles_x; a.tiles_z = tiles_z; a.ntiles = ntiles;
+        dim3 grid(ntiles, 1, p->n_shots);
+        HIP_TRY(hipEventRecord(e0, st));
+        for (int r = 0; r < reps; ++r) {
+            if (which == 2) hipLaunchKernelGGL((k_adj_strain<T, FD>), grid, blk, 0, st, a, it);
+            else hipLaunchKernelGGL((k_adj_vel<T, FD>), grid, blk, 0, st, a, it);
+        }
+        HIP_TRY(hipEventRecord(e1, st));
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventSynchronize(e1));
+    float f = 0;
+    HIP_TRY(hipEventElapsedTime(&f, e0, e1));
+    *ms = f;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return 0;
+}
+
+extern "C" int stfwi_time_kernel(stfwi_plan *p, int which, int reps, void *stream, double *ms)
+{
+    if (!p || !ms || which < 0 || which > 3 || reps < 1) return fail(STFWI_ERR_INVALID, "bad argument");
+    if (!p->have_model || !p->n_shots) return fail(STFWI_ERR_STATE, "model or shots not set");
+    if (which == 3 && p->cfg.imaging && !p->fwd_done) return fail(STFWI_ERR_STATE, "needs a forward first");
+    hipStream_t st = (hipStream_t)stream;
+    const int fd = p->cfg.fd_order;
+    if (p->cfg.dtype == STFWI_F64)
+        return fd == 8 ? time_kernel_t<double, 8>(p, which, reps, st, ms)
+             : fd == 4 ? time_kernel_t<double, 4>(p, which, reps, st, ms) : time_kernel_t<double, 2>(p, which, reps, st, ms);
+    return fd == 8 ? time_kernel_t<float, 8>(p, which, reps, st, ms)
+         : fd == 4 ? time_kernel_t<float, 4>(p, which, reps, st, ms) : time_kernel_t<float, 2>(p, which, reps, st, ms);
+}

@@ -1,0 +1,142 @@
+"""Parity of the HIP path (through the C-ABI) with the CPU oracle, and the
+adjoint invariants, on an MI355X."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from oracle import signal as osig
+from tests.common import (DP_CONFIGS, TOL_DOT, TOL_F32, TOL_F64, dp_geometry, dp_model, dp_vectors, rel_l2)
+
+pytestmark = pytest.mark.gpu
+
+
+def _plan(m, fd, fs, dtype, nshots=1, nrec=8, imaging="none", grad="vs", hist=False, nt=None):
+    from stfwi_amd import Plan
+    return Plan(m["nz"], m["nx"], m["dx"], m["dz"], m["dt"], m["fc"], m["pad"], fd_order=fd, free_surface=fs,
+                dtype=dtype, n_shots_max=nshots, nt_max=nt or m["nt"], nrec_max=nrec, imaging=imaging,
+                grad=grad, store_history=hist)
+
+
+@pytest.mark.parametrize("fd,fs,st", DP_CONFIGS)
+def test_gpu_forward_adjoint_match_oracle_f64(fd, fs, st):
+    m = dp_model()
+    sx, sz, rx, rz = dp_geometry(m["nz"], m["nx"], fs, st)
+    s, y = dp_vectors(rx.size, m["nt"])
+    print(f"case fd={fd} fs={fs} src={st}", flush=True)
+    p = _plan(m, fd, fs, torch.float64, nrec=rx.size)
+    p.set_model(m["Vs"], m["Vp"], m["rho"])
+    p.set_shots([sx], [sz], st, m["nt"], rx, rz, s)
+    rec = p.forward()["rec"][0].cpu().numpy()
+    src_a = p.adjoint(torch.as_tensor(y[None]))["src_adj"][0].cpu().numpy()
+    g = oracle.Geom(m["nz"], m["nx"], m["pad"], fs, fd, m["nt"], m["dx"], m["dz"], m["dt"], m["fc"])
+    rec_o, _ = oracle.forward(g, m["Vs"], m["Vp"], m["rho"], sx, sz, s, rx, rz, st)
+    a_o = oracle.adjoint(g, m["Vs"], m["Vp"], m["rho"], sx, sz, y, rx, rz, st)
+    assert rel_l2(rec, rec_o) < TOL_F64
+    assert rel_l2(src_a, a_o["src_a"]) < TOL_F64
+    # dot-product test on the GPU pair itself (test_adjoint_kernel.py:100-108)
+    lhs, rhs = float(np.sum(rec * y)), float(np.sum(s * src_a))
+    assert abs(lhs - rhs) / max(abs(lhs), abs(rhs)) < TOL_DOT
+
+
+def _grad_case(fd, fs, st, seed=0):
+    m = dp_model()
+    m["nt"] = 120
+    rng = np.random.default_rng(seed)
+    sx, sz, rx, rz = dp_geometry(m["nz"], m["nx"], fs, st)
+    wav = osig.ricker(np.arange(m["nt"]) * m["dt"], m["fc"], 1.5 / m["fc"]) * 1e6
+    obs = rng.standard_normal((m["nt"], rx.size)) * 1e-9
+    return m, sx, sz, rx, rz, wav, obs
+
+
+@pytest.mark.parametrize("imaging", ["exact", "zhou"])
+@pytest.mark.parametrize("fd,fs,st", DP_CONFIGS)
+def test_gpu_gradient_matches_oracle_f64(imaging, fd, fs, st):
+    m, sx, sz, rx, rz, wav, obs = _grad_case(fd, fs, st)
+    print(f"case {imaging} fd={fd} fs={fs} src={st}", flush=True)
+    sos = np.array([[0.1, 0.2, 0.1, 1.0, -1.2, 0.5]])
+    comp, ntap, scale = 2, 12, 1e15
+    p = _plan(m, fd, fs, torch.float64, nrec=rx.size, imaging=imaging, grad="all", hist=True)
+    p.set_model(m["Vs"], m["Vp"], m["rho"])
+    p.set_shots([sx], [sz], st, m["nt"], rx, rz, wav)
+    rec = p.forward()["rec"]
+    loss, radj = p.residual(rec, torch.as_tensor(obs[None]), comp, sos, ntap, scale, 1)
+    p.adjoint(radj)
+    gg = p.finalize(m["Vs"], m["Vp"], m["rho"]).cpu().numpy()
+    g = oracle.Geom(m["nz"], m["nx"], m["pad"], fs, fd, m["nt"], m["dx"], m["dz"], m["dt"], m["fc"])
+    tape = ("e", "uv") if imaging == "zhou" else ("e", "dxu0", "s")
+    rec_o, t = oracle.forward(g, m["Vs"], m["Vp"], m["rho"], sx, sz, wav, rx, rz, st, np.float64, tape)
+    l_o, cot = osig.residual_prep(rec_o[comp], obs, sos, ntap, scale, 1)
+    assert abs(float(loss[0]) / l_o - 1) < TOL_F64
+    ra = np.zeros_like(rec_o)
+    ra[comp] = cot
+    assert rel_l2(radj[0].cpu().numpy(), ra) < TOL_F64
+    a = oracle.adjoint(g, m["Vs"], m["Vp"], m["rho"], sx, sz, ra, rx, rz, st, wav, imaging, t)
+    chain = oracle.exact_chain if imaging == "exact" else oracle.zhou_chain
+    go = chain(g, a["g"], m["Vs"], m["Vp"], m["rho"])
+    for i in range(3):
+        assert rel_l2(gg[i], go[i]) < TOL_F64, i
+
+
+def test_gpu_batched_shots_equal_single():
+    """Several shots with uneven nt and receiver spreads in one launch give
+    the same results as one shot at a time (bitwise)."""
+    m = dp_model()
+    m["nt"] = 90
+    wav = osig.ricker(np.arange(m["nt"]) * m["dt"], m["fc"], 1.5 / m["fc"]) * 1e6
+    shots = [(5, 3, "force", 90, np.arange(2, 30, 3)), (20, 6, "moment", 60, np.arange(0, 32, 2)),
+             (11, 2, "moment", 75, np.array([4, 4, 9, 13]))]
+    nrec = max(len(s[4]) for s in shots)
+    pb = _plan(m, 8, True, torch.float32, nshots=3, nrec=nrec, imaging="exact", grad="vs", hist=True)
+    pb.set_model(m["Vs"], m["Vp"], m["rho"])
+    pb.set_shots([s[0] for s in shots], [s[1] for s in shots], [s[2] for s in shots], [s[3] for s in shots],
+                 [s[4] for s in shots], [np.zeros_like(s[4]) for s in shots], wav)
+    rb = pb.forward()["rec"]
+    rng = np.random.default_rng(1)
+    radj = torch.as_tensor(rng.standard_normal(rb.shape), dtype=torch.float32, device=rb.device)
+    for i, s in enumerate(shots):
+        radj[i, :, s[3]:] = 0
+        radj[i, :, :, len(s[4]):] = 0
+    sab = pb.adjoint(radj)["src_adj"]
+    gb = pb.finalize(m["Vs"], m["Vp"], m["rho"])
+    gsum = torch.zeros_like(gb)
+    for i, s in enumerate(shots):
+        p1 = _plan(m, 8, True, torch.float32, nshots=1, nrec=nrec, imaging="exact", grad="vs", hist=True)
+        p1.set_model(m["Vs"], m["Vp"], m["rho"])
+        p1.set_shots([s[0]], [s[1]], s[2], s[3], s[4], np.zeros_like(s[4]), wav)
+        r1 = p1.forward()["rec"]
+        assert torch.equal(r1[0], rb[i])
+        sa1 = p1.adjoint(radj[i:i + 1])["src_adj"]
+        assert torch.equal(sa1[0], sab[i])
+        gsum += p1.finalize(m["Vs"], m["Vp"], m["rho"])
+    assert rel_l2(gb.cpu().numpy(), gsum.cpu().numpy()) < 1e-6
+
+
+@pytest.mark.parametrize("fd", [4, 8])
+def test_gpu_f32_gradient_close_to_f64_oracle(fd):
+    """FP32 HIP path against the FP64 oracle on a 1000-step case."""
+    m = dp_model()
+    m["nt"] = 400
+    fs, st = True, "moment"
+    sx, sz, rx, rz = dp_geometry(m["nz"], m["nx"], fs, st)
+    wav = osig.ricker(np.arange(m["nt"]) * m["dt"], m["fc"], 1.5 / m["fc"])
+    g = oracle.Geom(m["nz"], m["nx"], m["pad"], fs, fd, m["nt"], m["dx"], m["dz"], m["dt"], m["fc"])
+    obs, _ = oracle.forward(g, m["Vs"] * 1.02, m["Vp"], m["rho"], sx, sz, wav, rx, rz, st)
+    obs = obs[2]
+    scale = 1.0 / np.mean(obs ** 2)
+    p = _plan(m, fd, fs, torch.float32, nrec=rx.size, imaging="exact", grad="vs", hist=True)
+    p.set_model(m["Vs"], m["Vp"], m["rho"])
+    p.set_shots([sx], [sz], st, m["nt"], rx, rz, wav)
+    rec = p.forward()["rec"]
+    loss, radj = p.residual(rec, torch.as_tensor(obs[None]), 2, None, 0, scale, 1)
+    p.adjoint(radj)
+    gg = p.finalize(m["Vs"], m["Vp"], m["rho"]).cpu().numpy()
+    rec_o, t = oracle.forward(g, m["Vs"], m["Vp"], m["rho"], sx, sz, wav, rx, rz, st, np.float64, ("e", "dxu0"))
+    l_o, cot = osig.residual_prep(rec_o[2], obs, None, 0, scale, 1)
+    ra = np.zeros_like(rec_o)
+    ra[2] = cot
+    a = oracle.adjoint(g, m["Vs"], m["Vp"], m["rho"], sx, sz, ra, rx, rz, st, wav, "exact", t)
+    go = oracle.exact_chain(g, a["g"], m["Vs"], m["Vp"], m["rho"])
+    assert rel_l2(rec[0].cpu().numpy(), rec_o) < TOL_F32
+    assert abs(float(loss[0]) / l_o - 1) < TOL_F32
+    assert rel_l2(gg[0], go[0]) < TOL_F32
