@@ -11,7 +11,6 @@ fused into each reverse step, shot reduction and chain rule to Vs.
 from __future__ import annotations
 
 import argparse
-import ctypes as C
 import json
 import os
 import sys
@@ -116,7 +115,6 @@ def main():
         sys.exit(2)
     import torch
     from stfwi_amd import Plan
-    from stfwi_amd import _abi
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
@@ -130,6 +128,8 @@ def main():
     obs = plan.forward(channels=(2,))["rec"][:, 2].contiguous()
     scale = float(1.0 / (torch.mean(obs.double() ** 2) + 1e-45))
     vs0, vp0, rho0 = (torch.as_tensor(a, dtype=f32, device=dev) for a in sv["init"])
+    plan.set_model(vs0, vp0, rho0)
+    tiles, tune = plan.autotune(20)
 
     def step():
         plan.set_model(vs0, vp0, rho0)
@@ -153,20 +153,12 @@ def main():
 
     # dominant kernel timed live: the velocity-adjoint launches of one
     # reverse time loop, bracketed with HIP events on the plan's stream
-    L = _abi.lib()
-    kms = C.c_double()
-    kern = {}
-    if hasattr(L, "stfwi_time_kernel"):
-        L.stfwi_time_kernel.restype = C.c_int
-        L.stfwi_time_kernel.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_double)]
-        reps = 200
-        _abi.check(L.stfwi_time_kernel(plan._h, 3, reps, C.c_void_p(torch.cuda.current_stream().cuda_stream),
-                                       C.byref(kms)))
-        avg_s = kms.value / 1e3 / reps
-        kb = dominant_kernel_bytes(4, 1)
-        kern = {"bound": "hbm", "achieved": kb / avg_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": kb / avg_s / HBM_PEAK, "traffic": None, "kernel": "k_adj_vel<float,8>",
-                "avg_launch_us": avg_s * 1e6, "algorithmic_bytes_per_launch": kb}
+    per_kernel = {n: plan.time_kernel(i, 200) for i, n in enumerate(("fwd_vel", "fwd_strain", "adj_strain", "adj_vel"))}
+    avg_s = per_kernel["adj_vel"] * 1e-6
+    kb = dominant_kernel_bytes(4, 1)
+    kern = {"bound": "hbm", "achieved": kb / avg_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": kb / avg_s / HBM_PEAK, "traffic": None, "kernel": f"k_adj_vel<float,8,{tiles[3]}>",
+            "avg_launch_us": avg_s * 1e6, "algorithmic_bytes_per_launch": kb}
     tot, _, _ = algorithmic_bytes_per_shot_gradient(4, 1)
     path = {"algorithmic_GB_per_shot_gradient": tot / 1e9, "achieved_GB_s": value * tot / 1e9,
             "frac_of_hbm_peak": value * tot / HBM_PEAK}
@@ -180,7 +172,7 @@ def main():
            "config": {"workload": "25-shot gradient, 152x500 fd8 free surface pad 20, nt=3000, "
                                   "500 receivers, moment sources, exact imaging, Vs",
                       "shots": NSHOT, "nt": NT, "grid": [NZ, NX], "imaging": args.imaging},
-           "roofline": kern or None, "path_roofline": path, "cpu_baseline": cpu,
+           "roofline": kern, "kernel_us": per_kernel, "tiles": tiles, "autotune_us": tune, "path_roofline": path, "cpu_baseline": cpu,
            "finite": ok, "loss": float(loss.double().sum())}
     print(json.dumps(out))
 

@@ -144,12 +144,51 @@ int stfwi_residual(stfwi_plan *plan, const void *pred, const void *obs, int comp
                    const double *sos, int n_sos, int n_taper, double scale, int n_shots_norm,
                    void *loss_out, void *rec_adj, void *stream);
 
+/* Gaussian smoothing of a (nz, nx) array: normalised kernel of radius
+ * int(3 sigma + 0.5), 'same'-mode zero padding (gaussian_smooth_2d,
+ * fwi/loss.py:17-30).  tmp is an (nz, nx) workspace not aliasing in/out. */
+int stfwi_smooth2d(const void *in, void *tmp, void *out, int nz, int nx, double sigma, int dtype, void *stream);
+
+/* Illumination multiplier H = 1/(Ws + eps max Ws), normalised to max 1, from
+ * n_shots padded energies ws (n_shots x nz_t x nx_t) summed over shots and
+ * cropped to rows [z0, z0+nz_c), columns [pad, pad+nx_c)
+ * (compute_illumination_for_stage, fwi/illumination.py:38-48). */
+int stfwi_illumination(const void *ws, int n_shots, int nz_t, int nx_t, int z0, int pad, int nz_c, int nx_c,
+                       double eps, int dtype, void *out, void *stream);
+
+/* Plessix-Mulder multiplier from summed padded energies: We (eprecond3,
+ * fwi/preconditioning.py:65-139) into we_pad (nz_t x nx_t), then the
+ * normalised physical-grid multiplier 1/We / max into out (nz_dom x nx_dom),
+ * as fwi/optim_lbfgsb.py:90-100.  rec_x/rec_z are host arrays. */
+int stfwi_eprecond3(const void *ws, int n_shots, int nz_t, int nx_t, int free_surface, int pad, int nz_dom,
+                    int nx_dom, const int *rec_x, const int *rec_z, int nrec, double dx, double dz, double eps,
+                    int dtype, void *we_pad, void *out, void *stream);
+
+/* In-place trace filtering x[t][trace] (nt x ntr): taper the last n_taper
+ * samples then the DF-I biquad cascade (jax_sosfilt / cosine_taper_end,
+ * fwi/filters.py:14-42), or with transpose=1 the exact transpose of that
+ * map.  sos is a host (n_sos x 6) array, n_sos <= 8. */
+int stfwi_sosfilt(void *x, int nt, int ntr, const double *sos, int n_sos, int n_taper, int transpose, int dtype,
+                  void *stream);
+
 /* Diagnostics (no reference counterpart): time `reps` back-to-back launches
  * of one time-step kernel over all shots of the plan (which: 0 forward
  * velocity, 1 forward strain, 2 adjoint strain, 3 adjoint velocity with
  * imaging) with HIP events on `stream`; *ms receives the total.  Host call,
  * synchronous.  It overwrites the plan's wavefield state. */
 int stfwi_time_kernel(stfwi_plan *plan, int which, int reps, void *stream, double *ms);
+
+/* Tile-shape autotuner (no reference counterpart): for each of the four
+ * time-step kernels time `reps` launches at every compiled tile height
+ * (8/16/32 rows of 64 columns in FP32, 8 in FP64) on the plan's shots and
+ * keep the fastest.  tz_out (4 ints) and us_out (4 x 3 average microseconds,
+ * 0 where not compiled) may be NULL.  Host call, synchronous; overwrites the
+ * wavefield state, so run it before stfwi_forward. */
+int stfwi_autotune(stfwi_plan *plan, int reps, void *stream, int *tz_out, double *us_out);
+
+/* Set the four tile heights explicitly (kernel order: forward velocity,
+ * forward strain, adjoint strain, adjoint velocity). */
+int stfwi_set_tiles(stfwi_plan *plan, const int *tz);
 
 #ifdef __cplusplus
 }

@@ -31,6 +31,15 @@ SIGNATURES = {
     "stfwi_forward": (C.c_int, [_VP, _VP, C.c_int, _VP, C.c_int, _VP, C.c_int, _VP]),
     "stfwi_adjoint": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int, _VP]),
     "stfwi_gradient_finalize": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int, _VP, _VP]),
+    "stfwi_time_kernel": (C.c_int, [_VP, C.c_int, C.c_int, _VP, C.POINTER(C.c_double)]),
+    "stfwi_autotune": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP]),
+    "stfwi_set_tiles": (C.c_int, [_VP, _VP]),
+    "stfwi_smooth2d": (C.c_int, [_VP, _VP, _VP, C.c_int, C.c_int, C.c_double, C.c_int, _VP]),
+    "stfwi_illumination": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.c_double, C.c_int, _VP, _VP]),
+    "stfwi_eprecond3": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _VP, _VP,
+                                  C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _VP, _VP, _VP]),
+    "stfwi_sosfilt": (C.c_int, [_VP, C.c_int, C.c_int, _VP, C.c_int, C.c_int, C.c_int, C.c_int, _VP]),
     "stfwi_residual": (C.c_int, [_VP, _VP, _VP, C.c_int, _VP, C.c_int, C.c_int, C.c_double, C.c_int,
                                  _VP, _VP, _VP]),
 }

@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace stfwi {
 
 typedef long long i64;
@@ -211,19 +213,57 @@ struct FwdArgs {
     int tiles_x, tiles_z, ntiles;
 };
 
-#define FWD_TILE_X 64
-#define FWD_TILE_Z 4
+#define TILE_X 64
+#define BLOCK_Y 4
+
+// Every time-step kernel works on a TZ x TILE_X tile (TZ = 8, 16 or 32, a
+// template parameter the autotuner picks) with 256 threads (64 x 4, each
+// thread owns TZ/4 rows).  Phase 1 stages what the stencils read (stress
+// products, velocities, or the C-PML-transposed derivative adjoints) for
+// the tile plus a 4-cell halo in LDS, computing each staged value once;
+// phase 2 evaluates the per-term-masked stencils from LDS.  Tiles whose
+// 8-cell neighbourhood is away from every grid edge, PML strip and the
+// free-surface row run a mask-free instantiation of the same body.
+
+// XCD-aware block order: workgroup n is dispatched to XCD n % 8, so give
+// each XCD a contiguous range of logical blocks; neighbouring tiles (which
+// share halos) and whole shots then stay in one XCD's L2.  The map is a
+// bijection on [0, gridDim.x * gridDim.z) whatever the dispatch order.
+__device__ __forceinline__ void logical_block(int &bx, int &bz)
+{
+    const int gx = gridDim.x;
+    const int n = gx * gridDim.z;
+    const int L = blockIdx.x + gx * blockIdx.z;
+    const int q = n >> 3, rmd = n & 7;
+    const int xcd = L & 7, w = L >> 3;
+    const int lg = xcd < rmd ? xcd * (q + 1) + w : rmd * (q + 1) + (xcd - rmd) * q + w;
+    bz = lg / gx;
+    bx = lg - bz * gx;
+}
+
+// tile [r0, r0+TZ) x [c0, c0+64) is "interior" when every cell within 8 of
+// it is inside the grid, off the PML strips and off the free-surface row:
+// all per-term masks are then true and the C-PML recursions are no-ops
+template <typename T>
+__device__ __forceinline__ bool interior_tile(const Geo<T> &g, int r0, int c0, int tz)
+{
+    const int top = g.fs ? 8 : (g.pad + 8 > 8 ? g.pad + 8 : 8);
+    return r0 >= top && r0 + tz + 8 <= g.nz - g.pad && c0 >= g.pad + 8 && c0 + TILE_X + 8 <= g.nx - g.pad;
+}
+
+#define CND(x) (INT || (x))
 
 // velocity half-step (fwi/forward.py:253-367) + exx/ezz receivers of the
 // previous step (strains are read-only here)
-template <typename T, int FD>
+template <typename T, int FD, int TZ>
 __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
 {
     const Geo<T> &g = a.g;
-    const int s = blockIdx.z;
+    int bx, s;
+    logical_block(bx, s);
     const ShotDev sh = a.shots[s];
-    if ((int)blockIdx.x >= a.ntiles) {
-        const int j = ((int)blockIdx.x - a.ntiles) * 256 + threadIdx.y * FWD_TILE_X + threadIdx.x;
+    if (bx >= a.ntiles) {
+        const int j = (bx - a.ntiles) * 256 + threadIdx.y * TILE_X + threadIdx.x;
         const int itp = it - 1;
         if (itp < 0 || itp >= sh.nt || j >= sh.nrec || !a.rec) return;
         const i64 cell = g.origin + a.rec_cell[sh.rec_off + j];
@@ -234,11 +274,15 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
         return;
     }
     if (it >= sh.nt) return;
-    const int tz = blockIdx.x / a.tiles_x, tx = blockIdx.x - tz * a.tiles_x;
-    const int r = tz * FWD_TILE_Z + threadIdx.y, c = tx * FWD_TILE_X + threadIdx.x;
-    if (r > g.nz - 2 || c < 1 || c > g.nx - 2) return;
+    constexpr int RPT = TZ / BLOCK_Y;
+    constexpr int NXH = TZ * (TILE_X + 8), NZH = (TZ + 8) * TILE_X;
+    __shared__ T sP1[TZ][TILE_X + 8], sP2[TZ][TILE_X + 8], sP3x[TZ][TILE_X + 8];
+    __shared__ T sP3z[TZ + 8][TILE_X], sP4[TZ + 8][TILE_X], sP5[TZ + 8][TILE_X];
     constexpr int NK = FDC<FD>::NK;
     const int ld = g.ld, nz = g.nz, nx = g.nx;
+    const int tz = bx / a.tiles_x, tx = bx - tz * a.tiles_x;
+    const int r0 = tz * TZ, c0 = tx * TILE_X;
+    const int tid = threadIdx.y * TILE_X + threadIdx.x;
     T *st = a.state + s * a.sstride + g.origin;
     T *__restrict__ U = st;
     T *__restrict__ V = st + g.fstride;
@@ -248,77 +292,126 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
     const T *__restrict__ L2M = a.L2M + g.origin;
     const T *__restrict__ M = a.M + g.origin;
     const T *__restrict__ B = a.B + g.origin;
-    const i64 q = (i64)r * ld + c;
-    auto P1 = [&](i64 p) { return L2M[p] * ex[p]; };                       // (lambda+2mu) exx
-    auto P2 = [&](i64 p) { return (L2M[p] - T(2) * M[p]) * ez[p]; };       // lambda ezz
-    auto P3 = [&](i64 p) { return T(2) * M[p] * es[p]; };                  // 2 mu exz
-    auto P4 = [&](i64 p) { return L2M[p] * ez[p]; };
-    auto P5 = [&](i64 p) { return (L2M[p] - T(2) * M[p]) * ex[p]; };
-    T D1 = 0, D2 = 0, D3 = 0, D4 = 0, D5 = 0, D6 = 0;
+    // stress products: (lambda+2mu) exx, lambda ezz, 2 mu exz along x;
+    // 2 mu exz, (lambda+2mu) ezz, lambda exx along z
 #pragma unroll
-    for (int k = 1; k <= NK; ++k) {
-        const T ck = (T)fdc<FD>(k);
-        // D+x, term range [k-1, nx-1-k] (term 1: [0, nx-2], always true here)
-        if (c >= k - 1 && c <= nx - 1 - k) {
-            D1 += ck * (P1(q + k) - P1(q - k + 1));
-            D2 += ck * (P2(q + k) - P2(q - k + 1));
-        }
-        // D-z of 2 mu exz, term 1 rows [1, nz-2], term k rows [k, nz-k]
-        if (k == 1 ? (r >= 1 && r <= nz - 2) : (r >= k && r <= nz - k))
-            D3 += ck * (P3(q + (i64)(k - 1) * ld) - P3(q - (i64)k * ld));
-        // D-x of 2 mu exz, term 1 cols [1, nx-2], term k cols [k, nx-k]
-        if (k == 1 || (c >= k && c <= nx - k)) D4 += ck * (P3(q + k - 1) - P3(q - k));
-        // D+z, term range [k-1, nz-1-k]
-        if (r >= k - 1 && r <= nz - 1 - k) {
-            D5 += ck * (P4(q + (i64)k * ld) - P4(q - (i64)(k - 1) * ld));
-            D6 += ck * (P5(q + (i64)k * ld) - P5(q - (i64)(k - 1) * ld));
-        }
-    }
-    if (g.fs && r == 0) D3 = T(2 * fdc<FD>(1)) * P3(q);   // image of sigma_xz at the surface
-    const int ix = xstrip(g, r, c), iz = zstrip(g, r, c);
-    if (ix >= 0) {
-        T *px = a.psix + s * a.psix_stride + ix;
-        D1 = pml_fwd(D1, px + 0 * a.psix_n, a.profx, c, nx);
-        D2 = pml_fwd(D2, px + 1 * a.psix_n, a.profx, c, nx);
-        D4 = pml_fwd(D4, px + 2 * a.psix_n, a.profx, c, nx);
-    }
-    if (iz >= 0) {
-        T *pz = a.psiz + s * a.psiz_stride + iz;
-        D3 = pml_fwd(D3, pz + 0 * a.psiz_n, a.profz, r, nz);
-        D5 = pml_fwd(D5, pz + 1 * a.psiz_n, a.profz, r, nz);
-        D6 = pml_fwd(D6, pz + 2 * a.psiz_n, a.profz, r, nz);
-    }
-    const T b = B[q];
-    U[q] = U[q] + g.dtdx * b * D1 + g.dtdx * b * D2 + g.dtdz * b * D3;
-    T v = V[q] + g.dtdx * b * D4 + g.dtdz * b * D5 + g.dtdz * b * D6;
-    T sv = 0;
-    if (sh.type == 0 && r == sh.sz && c == sh.sx) {
-        const T w = a.wav[(i64)s * a.nt_max + it];
-        v += g.fsrc * b * w;
-        sv = g.fsrc * w;
-    }
-    V[q] = v;
-    if (a.hist2_mode == 1) {
-        const int rp = r - g.z0, cp = c - g.pad;
-        if (rp >= 0 && rp < g.nzd && cp >= 0 && cp < g.nxd) {
-            T *h = a.hist2 + ((i64)s * a.nt_max + it) * 2 * g.np + (i64)rp * g.nxd + cp;
-            h[0] = g.dtdx * D1 + g.dtdx * D2 + g.dtdz * D3;
-            h[g.np] = g.dtdx * D4 + g.dtdz * D5 + g.dtdz * D6 + sv;
+    for (int k = 0; k < (NXH + 255) / 256; ++k) {
+        const int i = tid + 256 * k;
+        if (NXH % 256 == 0 || i < NXH) {
+            const int zi = i / (TILE_X + 8), xi = i - zi * (TILE_X + 8);
+            const int r = r0 + zi, c = c0 - 4 + xi;
+            T p1 = 0, p2 = 0, p3 = 0;
+            if (r < nz && c >= 0 && c < nx) {
+                const i64 p = (i64)r * ld + c;
+                const T l2m = L2M[p], mu = M[p];
+                p1 = l2m * ex[p];
+                p2 = (l2m - T(2) * mu) * ez[p];
+                p3 = T(2) * mu * es[p];
+            }
+            sP1[zi][xi] = p1;
+            sP2[zi][xi] = p2;
+            sP3x[zi][xi] = p3;
         }
     }
+#pragma unroll
+    for (int k = 0; k < NZH / 256; ++k) {
+        const int i = tid + 256 * k;
+        const int zi = i / TILE_X, xi = i - zi * TILE_X;
+        const int r = r0 - 4 + zi, c = c0 + xi;
+        T p3 = 0, p4 = 0, p5 = 0;
+        if (r >= 0 && r < nz && c < nx) {
+            const i64 p = (i64)r * ld + c;
+            const T l2m = L2M[p], mu = M[p];
+            p3 = T(2) * mu * es[p];
+            p4 = l2m * ez[p];
+            p5 = (l2m - T(2) * mu) * ex[p];
+        }
+        sP3z[zi][xi] = p3;
+        sP4[zi][xi] = p4;
+        sP5[zi][xi] = p5;
+    }
+    __syncthreads();
+    const int lx = threadIdx.x, c = c0 + lx;
+    auto body = [&](auto tag) {
+        constexpr bool INT = decltype(tag)::value;
+#pragma unroll
+        for (int rr = 0; rr < RPT; ++rr) {
+            const int lz = threadIdx.y + BLOCK_Y * rr, r = r0 + lz;
+            if (!INT && (r > nz - 2 || c < 1 || c > nx - 2)) continue;
+            const i64 q = (i64)r * ld + c;
+            const int xc = lx + 4, zc = lz + 4;
+            T D1 = 0, D2 = 0, D3 = 0, D4 = 0, D5 = 0, D6 = 0;
+#pragma unroll
+            for (int k = 1; k <= NK; ++k) {
+                const T ck = (T)fdc<FD>(k);
+                // D+x, term range [k-1, nx-1-k] (term 1: [0, nx-2], always true here)
+                if (CND(c >= k - 1 && c <= nx - 1 - k)) {
+                    D1 += ck * (sP1[lz][xc + k] - sP1[lz][xc - k + 1]);
+                    D2 += ck * (sP2[lz][xc + k] - sP2[lz][xc - k + 1]);
+                }
+                // D-z of 2 mu exz, term 1 rows [1, nz-2], term k rows [k, nz-k]
+                if (CND(k == 1 ? (r >= 1 && r <= nz - 2) : (r >= k && r <= nz - k)))
+                    D3 += ck * (sP3z[zc + k - 1][lx] - sP3z[zc - k][lx]);
+                // D-x of 2 mu exz, term 1 cols [1, nx-2], term k cols [k, nx-k]
+                if (CND(k == 1 || (c >= k && c <= nx - k))) D4 += ck * (sP3x[lz][xc + k - 1] - sP3x[lz][xc - k]);
+                // D+z, term range [k-1, nz-1-k]
+                if (CND(r >= k - 1 && r <= nz - 1 - k)) {
+                    D5 += ck * (sP4[zc + k][lx] - sP4[zc - k + 1][lx]);
+                    D6 += ck * (sP5[zc + k][lx] - sP5[zc - k + 1][lx]);
+                }
+            }
+            if (!INT) {
+                if (g.fs && r == 0) D3 = T(2 * fdc<FD>(1)) * sP3z[zc][lx];   // image of sigma_xz at the surface
+                const int ix = xstrip(g, r, c), iz = zstrip(g, r, c);
+                if (ix >= 0) {
+                    T *px = a.psix + s * a.psix_stride + ix;
+                    D1 = pml_fwd(D1, px + 0 * a.psix_n, a.profx, c, nx);
+                    D2 = pml_fwd(D2, px + 1 * a.psix_n, a.profx, c, nx);
+                    D4 = pml_fwd(D4, px + 2 * a.psix_n, a.profx, c, nx);
+                }
+                if (iz >= 0) {
+                    T *pz = a.psiz + s * a.psiz_stride + iz;
+                    D3 = pml_fwd(D3, pz + 0 * a.psiz_n, a.profz, r, nz);
+                    D5 = pml_fwd(D5, pz + 1 * a.psiz_n, a.profz, r, nz);
+                    D6 = pml_fwd(D6, pz + 2 * a.psiz_n, a.profz, r, nz);
+                }
+            }
+            const T b = B[q];
+            U[q] = U[q] + g.dtdx * b * D1 + g.dtdx * b * D2 + g.dtdz * b * D3;
+            T v = V[q] + g.dtdx * b * D4 + g.dtdz * b * D5 + g.dtdz * b * D6;
+            T sv = 0;
+            if (sh.type == 0 && r == sh.sz && c == sh.sx) {
+                const T w = a.wav[(i64)s * a.nt_max + it];
+                v += g.fsrc * b * w;
+                sv = g.fsrc * w;
+            }
+            V[q] = v;
+            if (a.hist2_mode == 1) {
+                const int rp = r - g.z0, cp = c - g.pad;
+                if (CND(rp >= 0 && rp < g.nzd && cp >= 0 && cp < g.nxd)) {
+                    T *h = a.hist2 + ((i64)s * a.nt_max + it) * 2 * g.np + (i64)rp * g.nxd + cp;
+                    h[0] = g.dtdx * D1 + g.dtdx * D2 + g.dtdz * D3;
+                    h[g.np] = g.dtdx * D4 + g.dtdz * D5 + g.dtdz * D6 + sv;
+                }
+            }
+        }
+    };
+    if (interior_tile(g, r0, c0, TZ)) body(std::true_type{});
+    else body(std::false_type{});
 }
 
 // strain half-step (fwi/forward.py:369-444) + vx/vz receivers, history,
 // illumination energy, snapshots
-template <typename T, int FD>
+template <typename T, int FD, int TZ>
 __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
 {
     const Geo<T> &g = a.g;
-    const int s = blockIdx.z;
+    int bx, s;
+    logical_block(bx, s);
     const ShotDev sh = a.shots[s];
     if (it >= sh.nt) return;
-    if ((int)blockIdx.x >= a.ntiles) {
-        const int j = ((int)blockIdx.x - a.ntiles) * 256 + threadIdx.y * FWD_TILE_X + threadIdx.x;
+    if (bx >= a.ntiles) {
+        const int j = (bx - a.ntiles) * 256 + threadIdx.y * TILE_X + threadIdx.x;
         if (j >= sh.nrec || !a.rec) return;
         const i64 cell = g.origin + a.rec_cell[sh.rec_off + j];
         const T *st = a.state + s * a.sstride;
@@ -327,96 +420,130 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
         if (a.rec_mask & 2) ro[(i64)a.nt_max * a.nrec_max] = st[g.fstride + cell];
         return;
     }
-    const int tz = blockIdx.x / a.tiles_x, tx = blockIdx.x - tz * a.tiles_x;
-    const int r = tz * FWD_TILE_Z + threadIdx.y, c = tx * FWD_TILE_X + threadIdx.x;
-    if (r >= g.nz || c >= g.nx) return;
+    constexpr int RPT = TZ / BLOCK_Y;
+    constexpr int NB = (TZ + 8) * (TILE_X + 8);
+    __shared__ T sU[TZ + 8][TILE_X + 8], sV[TZ + 8][TILE_X + 8];
     constexpr int NK = FDC<FD>::NK;
     const int ld = g.ld, nz = g.nz, nx = g.nx;
+    const int tz = bx / a.tiles_x, tx = bx - tz * a.tiles_x;
+    const int r0 = tz * TZ, c0 = tx * TILE_X;
+    const int tid = threadIdx.y * TILE_X + threadIdx.x;
     T *st = a.state + s * a.sstride + g.origin;
     const T *__restrict__ U = st;
     const T *__restrict__ V = st + g.fstride;
     T *__restrict__ ex = st + 2 * g.fstride;
     T *__restrict__ ez = st + 3 * g.fstride;
     T *__restrict__ es = st + 4 * g.fstride;
-    const i64 q = (i64)r * ld + c;
-    const int ix = xstrip(g, r, c), iz = zstrip(g, r, c);
-    T *px = a.psix + s * a.psix_stride + ix;
-    T *pz = a.psiz + s * a.psiz_stride + iz;
-    // exx: D-x U on columns [1, nx-1]
-    T dxu = 0;
-    if (c >= 1) {
 #pragma unroll
-        for (int k = 1; k <= NK; ++k)
-            if (k == 1 || (c >= k && c <= nx - k))
-                dxu += (T)fdc<FD>(k) * (U[q + k - 1] - U[q - k]);
-    }
-    if (ix >= 0) dxu = pml_fwd(dxu, px + 3 * a.psix_n, a.profx, c, nx);
-    T exn = c >= 1 ? ex[q] + g.dtdx * dxu : ex[q];
-    // ezz: D-z V on columns [0, nx-2]; free-surface row from the C-PML'd D-x U
-    T dzv = 0;
-    if (c <= nx - 2) {
-        if (g.fs && r == 0) {
-            dzv = a.Lr0[c] * dxu * g.dz / g.dx;
-        } else {
-#pragma unroll
-            for (int k = 1; k <= NK; ++k)
-                if (k == 1 ? r >= 1 : (r >= k && r <= nz - k))
-                    dzv += (T)fdc<FD>(k) * (V[q + (i64)(k - 1) * ld] - V[q - (i64)k * ld]);
-        }
-    }
-    if (iz >= 0) dzv = pml_fwd(dzv, pz + 3 * a.psiz_n, a.profz, r, nz);
-    T ezn = ez[q] + g.dtdz * dzv;
-    if (sh.type == 1 && r == sh.sz && c == sh.sx) {
-        const T lm = (a.L2M[g.origin + q] - T(2) * a.M[g.origin + q]) + a.M[g.origin + q];
-        const T stm = a.wav[(i64)s * a.nt_max + it] * g.dt / (T(2) * lm * g.dx * g.dz);
-        exn += stm;
-        ezn += stm;
-    }
-    ex[q] = exn;
-    ez[q] = ezn;
-    // exz on [0, nz-2] x [0, nx-2]
-    T esn = es[q];
-    if (r <= nz - 2 && c <= nx - 2) {
-        T dzu = 0, dxv = 0;
-#pragma unroll
-        for (int k = 1; k <= NK; ++k) {
-            const T ck = (T)fdc<FD>(k);
-            if (c >= 1 && r >= k - 1 && r <= nz - 1 - k) dzu += ck * (U[q + (i64)k * ld] - U[q - (i64)(k - 1) * ld]);
-            if (c >= k - 1 && c <= nx - 1 - k) dxv += ck * (V[q + k] - V[q - k + 1]);
-        }
-        if (iz >= 0) dzu = pml_fwd(dzu, pz + 4 * a.psiz_n, a.profz, r, nz);
-        if (ix >= 0) dxv = pml_fwd(dxv, px + 4 * a.psix_n, a.profx, c, nx);
-        esn = esn + T(0.5) * (g.dtdz * dzu + g.dtdx * dxv);
-        es[q] = esn;
-    }
-    if (a.dxu0 && r == 0) a.dxu0[((i64)s * a.nt_max + it) * nx + c] = dxu;
-    const int rp = r - g.z0, cp = c - g.pad;
-    if (a.hist && rp >= 0 && rp < g.nzd && cp >= 0 && cp < g.nxd) {
-        const i64 o = ((i64)s * a.nt_max + it) * 3 * g.np + (i64)rp * g.nxd + cp;
-        a.hist[o] = exn;
-        a.hist[o + g.np] = ezn;
-        a.hist[o + 2 * g.np] = esn;
-        if (a.hist2_mode == 2) {
-            T *h = a.hist2 + ((i64)s * a.nt_max + it) * 2 * g.np + (i64)rp * g.nxd + cp;
-            h[0] = U[q];
-            h[g.np] = V[q];
-        }
-    }
-    const i64 pc = (i64)r * nx + c;
-    if (a.illum) {
-        const T f = a.illum_comp == 0 ? U[q] : a.illum_comp == 1 ? V[q] : a.illum_comp == 2 ? exn : ezn;
-        a.illum[(i64)s * nz * nx + pc] += f * f;
-    }
-    if (a.snap) {
-        const T vals[5] = {U[q], V[q], exn, ezn, esn};
-        int k = 0;
-        for (int f = 0; f < 5; ++f)
-            if (a.snap_mask & (1 << f)) {
-                a.snap[(((i64)s * a.nsnap + k) * a.nt_max + it) * nz * nx + pc] = vals[f];
-                ++k;
+    for (int k = 0; k < (NB + 255) / 256; ++k) {
+        const int i = tid + 256 * k;
+        if (NB % 256 == 0 || i < NB) {
+            const int zi = i / (TILE_X + 8), xi = i - zi * (TILE_X + 8);
+            const int r = r0 - 4 + zi, c = c0 - 4 + xi;
+            T u = 0, v = 0;
+            if (r >= 0 && r < nz && c >= 0 && c < nx) {
+                const i64 p = (i64)r * ld + c;
+                u = U[p];
+                v = V[p];
             }
+            sU[zi][xi] = u;
+            sV[zi][xi] = v;
+        }
     }
+    __syncthreads();
+    const int lx = threadIdx.x, c = c0 + lx;
+    auto body = [&](auto tag) {
+        constexpr bool INT = decltype(tag)::value;
+#pragma unroll
+        for (int rr = 0; rr < RPT; ++rr) {
+            const int lz = threadIdx.y + BLOCK_Y * rr, r = r0 + lz;
+            if (!INT && (r >= nz || c >= nx)) continue;
+            const int xc = lx + 4, zc = lz + 4;
+            const i64 q = (i64)r * ld + c;
+            const int ix = INT ? -1 : xstrip(g, r, c), iz = INT ? -1 : zstrip(g, r, c);
+            T *px = a.psix + s * a.psix_stride + ix;
+            T *pz = a.psiz + s * a.psiz_stride + iz;
+            // exx: D-x U on columns [1, nx-1]
+            T dxu = 0;
+            if (CND(c >= 1)) {
+#pragma unroll
+                for (int k = 1; k <= NK; ++k)
+                    if (CND(k == 1 || (c >= k && c <= nx - k)))
+                        dxu += (T)fdc<FD>(k) * (sU[zc][xc + k - 1] - sU[zc][xc - k]);
+            }
+            if (!INT && ix >= 0) dxu = pml_fwd(dxu, px + 3 * a.psix_n, a.profx, c, nx);
+            T exn = CND(c >= 1) ? ex[q] + g.dtdx * dxu : ex[q];
+            // ezz: D-z V on columns [0, nx-2]; free-surface row from the C-PML'd D-x U
+            T dzv = 0;
+            if (CND(c <= nx - 2)) {
+                if (!INT && g.fs && r == 0) {
+                    dzv = a.Lr0[c] * dxu * g.dz / g.dx;
+                } else {
+#pragma unroll
+                    for (int k = 1; k <= NK; ++k)
+                        if (CND(k == 1 ? r >= 1 : (r >= k && r <= nz - k)))
+                            dzv += (T)fdc<FD>(k) * (sV[zc + k - 1][xc] - sV[zc - k][xc]);
+                }
+            }
+            if (!INT && iz >= 0) dzv = pml_fwd(dzv, pz + 3 * a.psiz_n, a.profz, r, nz);
+            T ezn = ez[q] + g.dtdz * dzv;
+            if (sh.type == 1 && r == sh.sz && c == sh.sx) {
+                const T lm = (a.L2M[g.origin + q] - T(2) * a.M[g.origin + q]) + a.M[g.origin + q];
+                const T stm = a.wav[(i64)s * a.nt_max + it] * g.dt / (T(2) * lm * g.dx * g.dz);
+                exn += stm;
+                ezn += stm;
+            }
+            ex[q] = exn;
+            ez[q] = ezn;
+            // exz on [0, nz-2] x [0, nx-2]
+            T esn = es[q];
+            if (CND(r <= nz - 2 && c <= nx - 2)) {
+                T dzu = 0, dxv = 0;
+#pragma unroll
+                for (int k = 1; k <= NK; ++k) {
+                    const T ck = (T)fdc<FD>(k);
+                    if (CND(c >= 1 && r >= k - 1 && r <= nz - 1 - k)) dzu += ck * (sU[zc + k][xc] - sU[zc - k + 1][xc]);
+                    if (CND(c >= k - 1 && c <= nx - 1 - k)) dxv += ck * (sV[zc][xc + k] - sV[zc][xc - k + 1]);
+                }
+                if (!INT && iz >= 0) dzu = pml_fwd(dzu, pz + 4 * a.psiz_n, a.profz, r, nz);
+                if (!INT && ix >= 0) dxv = pml_fwd(dxv, px + 4 * a.psix_n, a.profx, c, nx);
+                esn = esn + T(0.5) * (g.dtdz * dzu + g.dtdx * dxv);
+                es[q] = esn;
+            }
+            if (!INT && a.dxu0 && r == 0) a.dxu0[((i64)s * a.nt_max + it) * nx + c] = dxu;
+            const int rp = r - g.z0, cp = c - g.pad;
+            if (a.hist && CND(rp >= 0 && rp < g.nzd && cp >= 0 && cp < g.nxd)) {
+                const i64 o = ((i64)s * a.nt_max + it) * 3 * g.np + (i64)rp * g.nxd + cp;
+                a.hist[o] = exn;
+                a.hist[o + g.np] = ezn;
+                a.hist[o + 2 * g.np] = esn;
+                if (a.hist2_mode == 2) {
+                    T *h = a.hist2 + ((i64)s * a.nt_max + it) * 2 * g.np + (i64)rp * g.nxd + cp;
+                    h[0] = sU[zc][xc];
+                    h[g.np] = sV[zc][xc];
+                }
+            }
+            const i64 pc = (i64)r * nx + c;
+            if (a.illum) {
+                const T f = a.illum_comp == 0 ? sU[zc][xc] : a.illum_comp == 1 ? sV[zc][xc] : a.illum_comp == 2 ? exn : ezn;
+                a.illum[(i64)s * nz * nx + pc] += f * f;
+            }
+            if (a.snap) {
+                const T vals[5] = {sU[zc][xc], sV[zc][xc], exn, ezn, esn};
+                int k = 0;
+                for (int f = 0; f < 5; ++f)
+                    if (a.snap_mask & (1 << f)) {
+                        a.snap[(((i64)s * a.nsnap + k) * a.nt_max + it) * nz * nx + pc] = vals[f];
+                        ++k;
+                    }
+            }
+        }
+    };
+    if (interior_tile(g, r0, c0, TZ)) body(std::true_type{});
+    else body(std::false_type{});
 }
+
+
 
 // ---------------------------------------------------------------------------
 // adjoint
@@ -522,82 +649,164 @@ struct K4Src {
 
 // strain-adjoint sweep: steps 0-T..2-T of fwi/adjoint_jax_kernel.py:201-323;
 // produces the final U_a, V_a of the step and the source-wavelet adjoint
-template <typename T, int FD>
+template <typename T, int FD, int TZ>
 __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
 {
     const Geo<T> &g = a.g;
-    const int s = blockIdx.z;
+    int bx, s;
+    logical_block(bx, s);
     const ShotDev sh = a.shots[s];
     if (it >= sh.nt) return;
-    const int tz = blockIdx.x / a.tiles_x, tx = blockIdx.x - tz * a.tiles_x;
-    const int r = tz * FWD_TILE_Z + threadIdx.y, c = tx * FWD_TILE_X + threadIdx.x;
-    if (r >= g.nz || c >= g.nx) return;
+    constexpr int RPT = TZ / BLOCK_Y;
+    constexpr int NXH = TZ * (TILE_X + 8), NZH = (TZ + 8) * TILE_X;
+    __shared__ T szU[TZ + 8][TILE_X], szV[TZ + 8][TILE_X];
+    __shared__ T sxV[TZ][TILE_X + 8], sxU[TZ][TILE_X + 8];
     constexpr int NK = FDC<FD>::NK;
     const int nz = g.nz, nx = g.nx;
+    const int tz = bx / a.tiles_x, tx = bx - tz * a.tiles_x;
+    const int r0 = tz * TZ, c0 = tx * TILE_X;
+    const int tid = threadIdx.y * TILE_X + threadIdx.x;
     T *st = a.state + s * a.sstride + g.origin;
     T *Ua = st, *Va = st + g.fstride;
     const K4Src<T> S{g, st + 4 * g.fstride, st + 2 * g.fstride, st + 3 * g.fstride,
                      a.psix_o + s * a.psix_stride, a.psiz_o + s * a.psiz_stride,
                      a.psix_len, a.psiz_len, a.profx, a.profz, a.Lr0};
-    const i64 q = (i64)r * g.ld + c;
-    T ua = Ua[q], va = Va[q];
+    const bool inter = interior_tile(g, r0, c0, TZ);
+    const int lx = threadIdx.x, c = c0 + lx;
+    // owned values, loaded ahead of the barrier
+    T ua0[RPT], va0[RPT];
 #pragma unroll
-    for (int k = 1; k <= NK; ++k) {
-        const T ck = (T)fdc<FD>(k);
-        // (D+z)^T of Dz_U_a, columns [1, nx-1]; term range [k-1, nz-1-k]
-        if (c >= 1) {
-            if (r - k >= k - 1 && r - k <= nz - 1 - k) ua += ck * S.DzUa(r - k, c);
-            if (r + k - 1 >= k - 1 && r + k - 1 <= nz - 1 - k) ua += -ck * S.DzUa(r + k - 1, c);
-        }
-        // (D+x)^T of Dx_V_a, rows [0, nz-2]
-        if (r <= nz - 2) {
-            if (c - k >= k - 1 && c - k <= nx - 1 - k) va += ck * S.DxVa(r, c - k);
-            if (c + k - 1 >= k - 1 && c + k - 1 <= nx - 1 - k) va += -ck * S.DxVa(r, c + k - 1);
-        }
+    for (int rr = 0; rr < RPT; ++rr) {
+        const int r = r0 + threadIdx.y + BLOCK_Y * rr;
+        const bool ok = r < nz && c < nx;
+        ua0[rr] = ok ? Ua[(i64)r * g.ld + c] : T(0);
+        va0[rr] = ok ? Va[(i64)r * g.ld + c] : T(0);
     }
+    // derivative adjoints after the transposed C-PML, tile + halo
+    if (inter) {
+        // no PML, no masks: the adjoints are the scaled strain adjoints
 #pragma unroll
-    for (int k = 1; k <= NK; ++k) {
-        const T ck = (T)fdc<FD>(k);
-        // (D-z)^T of Dz_V_a, columns [0, nx-2]; term 1 rows [1, nz-1], k: [k, nz-k]
-        if (c <= nx - 2) {
-            const int i1 = r - k + 1, i2 = r + k;
-            if (k == 1 ? (i1 >= 1 && i1 <= nz - 1) : (i1 >= k && i1 <= nz - k)) va += ck * S.DzVa(i1, c);
-            if (k == 1 ? (i2 >= 1 && i2 <= nz - 1) : (i2 >= k && i2 <= nz - k)) va += -ck * S.DzVa(i2, c);
+        for (int k = 0; k < NZH / 256; ++k) {
+            const int i = tid + 256 * k;
+            const int zi = i / TILE_X, xi = i - zi * TILE_X;
+            const i64 p = (i64)(r0 - 4 + zi) * g.ld + c0 + xi;
+            szU[zi][xi] = g.hdtdz * S.esa[p];
+            szV[zi][xi] = g.dtdz * S.eza[p];
         }
-    }
 #pragma unroll
-    for (int k = 1; k <= NK; ++k) {
-        const T ck = (T)fdc<FD>(k);
-        // (D-x)^T of Dx_U_a, all rows; term 1 cols [1, nx-1], k: [k, nx-k]
-        const int i1 = c - k + 1, i2 = c + k;
-        if (k == 1 ? (i1 >= 1 && i1 <= nx - 1) : (i1 >= k && i1 <= nx - k)) ua += ck * S.DxUa(r, i1);
-        if (k == 1 ? (i2 >= 1 && i2 <= nx - 1) : (i2 >= k && i2 <= nx - k)) ua += -ck * S.DxUa(r, i2);
-    }
-    Ua[q] = ua;
-    Va[q] = va;
-    // C-PML memories of this cell
-    const int ix = xstrip(g, r, c), iz = zstrip(g, r, c);
-    if (iz >= 0) {
-        T *pz = a.psiz_n + s * a.psiz_stride + iz;
-        pz[0] = pmlT_psi(S.GzU(r, c), S.pzo[iz], a.profz, r, nz);
-        pz[a.psiz_len] = pmlT_psi(S.GzV(r, c), S.pzo[a.psiz_len + iz], a.profz, r, nz);
-    }
-    if (ix >= 0) {
-        T *px = a.psix_n + s * a.psix_stride + ix;
-        px[0] = pmlT_psi(S.GxV(r, c), S.pxo[ix], a.profx, c, nx);
-        px[a.psix_len] = pmlT_psi(S.GxU(r, c), S.pxo[a.psix_len + ix], a.profx, c, nx);
-    }
-    if (r == sh.sz && c == sh.sx && a.src_adj) {
-        T sa;
-        if (sh.type == 1) {
-            const T lm = (a.L2M[g.origin + q] - T(2) * a.M[g.origin + q]) + a.M[g.origin + q];
-            sa = (g.dt / (T(2) * lm * g.dx * g.dz)) * (S.exa[q] + S.eza[q]);
-        } else {
-            sa = g.fsrc * a.B[g.origin + q] * va;
+        for (int k = 0; k < (NXH + 255) / 256; ++k) {
+            const int i = tid + 256 * k;
+            if (NXH % 256 == 0 || i < NXH) {
+                const int zi = i / (TILE_X + 8), xi = i - zi * (TILE_X + 8);
+                const i64 p = (i64)(r0 + zi) * g.ld + c0 - 4 + xi;
+                sxV[zi][xi] = g.hdtdx * S.esa[p];
+                sxU[zi][xi] = g.dtdx * S.exa[p];
+            }
         }
-        a.src_adj[(i64)s * a.nt_max + it] = sa;
+    } else {
+#pragma unroll
+        for (int k = 0; k < NZH / 256; ++k) {
+            const int i = tid + 256 * k;
+            const int zi = i / TILE_X, xi = i - zi * TILE_X;
+            const int r = r0 - 4 + zi, c1 = c0 + xi;
+            T u = 0, v = 0;
+            if (r >= 0 && r < nz && c1 < nx) {
+                u = S.DzUa(r, c1);
+                v = S.DzVa(r, c1);
+            }
+            szU[zi][xi] = u;
+            szV[zi][xi] = v;
+        }
+#pragma unroll
+        for (int k = 0; k < (NXH + 255) / 256; ++k) {
+            const int i = tid + 256 * k;
+            if (NXH % 256 == 0 || i < NXH) {
+                const int zi = i / (TILE_X + 8), xi = i - zi * (TILE_X + 8);
+                const int r = r0 + zi, c1 = c0 - 4 + xi;
+                T u = 0, v = 0;
+                if (r < nz && c1 >= 0 && c1 < nx) {
+                    v = S.DxVa(r, c1);
+                    u = S.DxUa(r, c1);
+                }
+                sxV[zi][xi] = v;
+                sxU[zi][xi] = u;
+            }
+        }
     }
+    __syncthreads();
+    auto body = [&](auto tag) {
+        constexpr bool INT = decltype(tag)::value;
+#pragma unroll
+        for (int rr = 0; rr < RPT; ++rr) {
+            const int lz = threadIdx.y + BLOCK_Y * rr, r = r0 + lz;
+            if (!INT && (r >= nz || c >= nx)) continue;
+            const int xc = lx + 4, zc = lz + 4;
+            const i64 q = (i64)r * g.ld + c;
+            T ua = ua0[rr], va = va0[rr];
+#pragma unroll
+            for (int k = 1; k <= NK; ++k) {
+                const T ck = (T)fdc<FD>(k);
+                // (D+z)^T of Dz_U_a, columns [1, nx-1]; term range [k-1, nz-1-k]
+                if (CND(c >= 1)) {
+                    if (CND(r - k >= k - 1 && r - k <= nz - 1 - k)) ua += ck * szU[zc - k][lx];
+                    if (CND(r + k - 1 >= k - 1 && r + k - 1 <= nz - 1 - k)) ua += -ck * szU[zc + k - 1][lx];
+                }
+                // (D+x)^T of Dx_V_a, rows [0, nz-2]
+                if (CND(r <= nz - 2)) {
+                    if (CND(c - k >= k - 1 && c - k <= nx - 1 - k)) va += ck * sxV[lz][xc - k];
+                    if (CND(c + k - 1 >= k - 1 && c + k - 1 <= nx - 1 - k)) va += -ck * sxV[lz][xc + k - 1];
+                }
+            }
+#pragma unroll
+            for (int k = 1; k <= NK; ++k) {
+                const T ck = (T)fdc<FD>(k);
+                // (D-z)^T of Dz_V_a, columns [0, nx-2]; term 1 rows [1, nz-1], k: [k, nz-k]
+                if (CND(c <= nx - 2)) {
+                    const int i1 = r - k + 1, i2 = r + k;
+                    if (CND(k == 1 ? (i1 >= 1 && i1 <= nz - 1) : (i1 >= k && i1 <= nz - k))) va += ck * szV[zc - k + 1][lx];
+                    if (CND(k == 1 ? (i2 >= 1 && i2 <= nz - 1) : (i2 >= k && i2 <= nz - k))) va += -ck * szV[zc + k][lx];
+                }
+            }
+#pragma unroll
+            for (int k = 1; k <= NK; ++k) {
+                const T ck = (T)fdc<FD>(k);
+                // (D-x)^T of Dx_U_a, all rows; term 1 cols [1, nx-1], k: [k, nx-k]
+                const int i1 = c - k + 1, i2 = c + k;
+                if (CND(k == 1 ? (i1 >= 1 && i1 <= nx - 1) : (i1 >= k && i1 <= nx - k))) ua += ck * sxU[lz][xc - k + 1];
+                if (CND(k == 1 ? (i2 >= 1 && i2 <= nx - 1) : (i2 >= k && i2 <= nx - k))) ua += -ck * sxU[lz][xc + k];
+            }
+            Ua[q] = ua;
+            Va[q] = va;
+            if (!INT) {
+                // C-PML memories of this cell
+                const int ix = xstrip(g, r, c), iz = zstrip(g, r, c);
+                if (iz >= 0) {
+                    T *pz = a.psiz_n + s * a.psiz_stride + iz;
+                    pz[0] = pmlT_psi(S.GzU(r, c), S.pzo[iz], a.profz, r, nz);
+                    pz[a.psiz_len] = pmlT_psi(S.GzV(r, c), S.pzo[a.psiz_len + iz], a.profz, r, nz);
+                }
+                if (ix >= 0) {
+                    T *px = a.psix_n + s * a.psix_stride + ix;
+                    px[0] = pmlT_psi(S.GxV(r, c), S.pxo[ix], a.profx, c, nx);
+                    px[a.psix_len] = pmlT_psi(S.GxU(r, c), S.pxo[a.psix_len + ix], a.profx, c, nx);
+                }
+            }
+            if (r == sh.sz && c == sh.sx && a.src_adj) {
+                T sa;
+                if (sh.type == 1) {
+                    const T lm = (a.L2M[g.origin + q] - T(2) * a.M[g.origin + q]) + a.M[g.origin + q];
+                    sa = (g.dt / (T(2) * lm * g.dx * g.dz)) * (S.exa[q] + S.eza[q]);
+                } else {
+                    sa = g.fsrc * a.B[g.origin + q] * va;
+                }
+                a.src_adj[(i64)s * a.nt_max + it] = sa;
+            }
+        }
+    };
+    if (inter) body(std::true_type{});
+    else body(std::false_type{});
 }
+
 
 // adjoints of the velocity-update derivatives (fwi/adjoint_jax_kernel.py:325-399)
 template <typename T>
@@ -656,170 +865,245 @@ struct K5Src {
 
 // velocity-adjoint sweep with the imaging condition fused: steps 2-T (V),
 // 1-T (U) and the constitutive transposes (fwi/adjoint_jax_kernel.py:325-464)
-template <typename T, int FD>
+template <typename T, int FD, int TZ>
 __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
 {
     const Geo<T> &g = a.g;
-    const int s = blockIdx.z;
+    int bx, s;
+    logical_block(bx, s);
     const ShotDev sh = a.shots[s];
     if (it >= sh.nt) return;
-    const int tz = blockIdx.x / a.tiles_x, tx = blockIdx.x - tz * a.tiles_x;
-    const int r = tz * FWD_TILE_Z + threadIdx.y, c = tx * FWD_TILE_X + threadIdx.x;
-    if (r >= g.nz || c >= g.nx) return;
+    constexpr int RPT = TZ / BLOCK_Y;
+    constexpr int NXH = TZ * (TILE_X + 8), NZH = (TZ + 8) * TILE_X;
+    __shared__ T sxBU[TZ][TILE_X + 8], sxMes[TZ][TILE_X + 8];
+    __shared__ T szBV[TZ + 8][TILE_X], szMes[TZ + 8][TILE_X];
     constexpr int NK = FDC<FD>::NK;
     const int nz = g.nz, nx = g.nx;
+    const int tz = bx / a.tiles_x, tx = bx - tz * a.tiles_x;
+    const int r0 = tz * TZ, c0 = tx * TILE_X;
+    const int tid = threadIdx.y * TILE_X + threadIdx.x;
     T *st = a.state + s * a.sstride + g.origin;
     const T *Ua = st, *Va = st + g.fstride;
     T *exa = st + 2 * g.fstride, *eza = st + 3 * g.fstride, *esa = st + 4 * g.fstride;
-    const K5Src<T> S{g, Ua, Va, a.B + g.origin, a.psix_o + s * a.psix_stride, a.psiz_o + s * a.psiz_stride,
+    const T *Bp = a.B + g.origin;
+    const K5Src<T> S{g, Ua, Va, Bp, a.psix_o + s * a.psix_stride, a.psiz_o + s * a.psiz_stride,
                      a.psix_len, a.psiz_len, a.profx, a.profz};
-    const i64 q = (i64)r * g.ld + c;
-    T Ax = 0, Az = 0, Mes = 0;
+    const bool inter = interior_tile(g, r0, c0, TZ);
+    const int lx = threadIdx.x, c = c0 + lx;
+    if (inter) {
+        // no PML, no masks: Dx_(BU)_a = dt/dx B U_a, Dz_M_es_a = dt/dz B U_a, ...
 #pragma unroll
-    for (int k = 1; k <= NK; ++k) {
-        const T ck = (T)fdc<FD>(k);
-        // (D+x)^T of Dx_(B U)_a (= L2M_ex_a = L_ez_a), rows [0, nz-2]
-        if (r <= nz - 2) {
-            if (c - k >= k - 1 && c - k <= nx - 1 - k) Ax += ck * S.DxBUa(r, c - k);
-            if (c + k - 1 >= k - 1 && c + k - 1 <= nx - 1 - k) Ax += -ck * S.DxBUa(r, c + k - 1);
-        }
-        // (D+z)^T of Dz_(B V)_a (= L2M_ez_a = L_ex_a), columns [1, nx-2]
-        if (c >= 1 && c <= nx - 2) {
-            if (r - k >= k - 1 && r - k <= nz - 1 - k) Az += ck * S.DzBVa(r - k, c);
-            if (r + k - 1 >= k - 1 && r + k - 1 <= nz - 1 - k) Az += -ck * S.DzBVa(r + k - 1, c);
-        }
-    }
-#pragma unroll
-    for (int k = 1; k <= NK; ++k) {
-        const T ck = (T)fdc<FD>(k);
-        // (D-x)^T of Dx_M_es_a, rows [0, nz-2]; term 1 cols [1, nx-2], k: [k, nx-k]
-        if (r <= nz - 2) {
-            const int i1 = c - k + 1, i2 = c + k;
-            if (k == 1 ? (i1 >= 1 && i1 <= nx - 2) : (i1 >= k && i1 <= nx - k)) Mes += ck * S.DxMesa(r, i1);
-            if (k == 1 ? (i2 >= 1 && i2 <= nx - 2) : (i2 >= k && i2 <= nx - k)) Mes += -ck * S.DxMesa(r, i2);
-        }
-    }
-#pragma unroll
-    for (int k = 1; k <= NK; ++k) {
-        const T ck = (T)fdc<FD>(k);
-        // (D-z)^T of Dz_M_es_a, columns [1, nx-2]; term 1 rows [1, nz-2], k: [k, nz-k]
-        if (c >= 1 && c <= nx - 2) {
-            const int i1 = r - k + 1, i2 = r + k;
-            if (k == 1 ? (i1 >= 1 && i1 <= nz - 2) : (i1 >= k && i1 <= nz - k)) Mes += ck * S.DzMesa(i1, c);
-            if (k == 1 ? (i2 >= 1 && i2 <= nz - 2) : (i2 >= k && i2 <= nz - k)) Mes += -ck * S.DzMesa(i2, c);
-        }
-    }
-    if (g.fs && r == 0 && c >= 1 && c <= nx - 2) Mes += T(2 * fdc<FD>(1)) * S.DzMesa(0, c);
-
-    const T l2m = a.L2M[g.origin + q], mu = a.M[g.origin + q];
-    const T lam = l2m - T(2) * mu;
-    const T exa0 = exa[q], eza0 = eza[q];
-    exa[q] = exa0 + l2m * Ax + lam * Az;
-    eza[q] = eza0 + lam * Ax + l2m * Az;
-    esa[q] = esa[q] + T(2) * mu * Mes;
-
-    const int ix = xstrip(g, r, c), iz = zstrip(g, r, c);
-    if (ix >= 0) {
-        T *px = a.psix_n + s * a.psix_stride + ix;
-        px[2 * a.psix_len] = pmlT_psi(S.GxMes(r, c), S.pxo[2 * a.psix_len + ix], a.profx, c, nx);
-        px[3 * a.psix_len] = pmlT_psi(S.GxBU(r, c), S.pxo[3 * a.psix_len + ix], a.profx, c, nx);
-    }
-    if (iz >= 0) {
-        T *pz = a.psiz_n + s * a.psiz_stride + iz;
-        pz[2 * a.psiz_len] = pmlT_psi(S.GzBV(r, c), S.pzo[2 * a.psiz_len + iz], a.profz, r, nz);
-        pz[3 * a.psiz_len] = pmlT_psi(S.GzMes(r, c), S.pzo[3 * a.psiz_len + iz], a.profz, r, nz);
-    }
-
-    if (a.snap) {
-        const T vals[5] = {Ua[q], Va[q], exa[q], eza[q], esa[q]};
-        int k = 0;
-        for (int f = 0; f < 5; ++f)
-            if (a.snap_mask & (1 << f)) {
-                a.snap[(((i64)s * a.nsnap + k) * a.nt_max + it) * nz * nx + (i64)r * nx + c] = vals[f];
-                ++k;
+        for (int k = 0; k < (NXH + 255) / 256; ++k) {
+            const int i = tid + 256 * k;
+            if (NXH % 256 == 0 || i < NXH) {
+                const int zi = i / (TILE_X + 8), xi = i - zi * (TILE_X + 8);
+                const i64 p = (i64)(r0 + zi) * g.ld + c0 - 4 + xi;
+                const T b = Bp[p];
+                sxBU[zi][xi] = g.dtdx * b * Ua[p];
+                sxMes[zi][xi] = g.dtdx * b * Va[p];
             }
-    }
-
-    // imaging on the physical cells
-    const int rp = r - g.z0, cp = c - g.pad;
-    if (a.imaging == 0 || rp < 0 || rp >= g.nzd || cp < 0 || cp >= g.nxd) return;
-    const i64 ip = (i64)rp * g.nxd + cp;
-    T *acc = a.gacc + (i64)s * a.nacc * g.np + ip;
-    const i64 hs = (i64)s * a.nt_max;
-    if (a.imaging == 2) {
-        // exact discrete gradient: model dependence of step it, paired with
-        // its carry-in strains (history frame it-1)
-        T gl2m = 0, gl = 0, gm = 0, gb = 0;
-        if (it > 0) {
-            const T *h = a.hist + (hs + it - 1) * 3 * g.np + ip;
-            const T exm = h[0], ezm = h[g.np], esm = h[2 * g.np];
-            gl2m = exm * Ax + ezm * Az;
-            gl = ezm * Ax + exm * Az;
-            gm = T(2) * esm * Mes;
         }
-        if (g.fs && r == 0 && c <= nx - 2) {
-            // sigma_zz = 0 row: Dz_V = L_ratio Dx_U dz/dx; row 0 has no z-PML
-            const T tt = g.dtdz * eza0 * a.dxu0[(hs + it) * nx + c] * g.dz / g.dx;
-            gl += -tt / l2m;
-            gl2m += tt * lam / (l2m * l2m);
-        }
-        if (sh.type == 1 && r == sh.sz && c == sh.sx) {
-            const T lm = lam + mu;
-            const T stm = a.wav[hs + it] * g.dt / (T(2) * lm * g.dx * g.dz);
-            const T Qm = -stm / lm * (exa0 + eza0);
-            gl += Qm;
-            gm += Qm;
-        }
-        if (a.gparams == 3) {
-            const T *h2 = a.hist2 + (hs + it) * 2 * g.np + ip;
-            gb = Ua[q] * h2[0] + Va[q] * h2[g.np];
-            acc[0] += gl2m;
-            acc[g.np] += gl;
-            acc[2 * g.np] += gm;
-            acc[3 * g.np] += gb;
-        } else {
-            acc[0] += gm - T(2) * gl;
-            if (a.gparams == 2) acc[g.np] += gl2m + gl;
+#pragma unroll
+        for (int k = 0; k < NZH / 256; ++k) {
+            const int i = tid + 256 * k;
+            const int zi = i / TILE_X, xi = i - zi * TILE_X;
+            const i64 p = (i64)(r0 - 4 + zi) * g.ld + c0 + xi;
+            const T b = Bp[p];
+            szBV[zi][xi] = g.dtdz * b * Va[p];
+            szMes[zi][xi] = g.dtdz * b * Ua[p];
         }
     } else {
-        // Zhou imaging (fwi/imaging.py:125-152): forward frame it with the
-        // adjoint carry-out of step it; centred periodic differences of B v_a
-        const T *h = a.hist + (hs + it) * 3 * g.np + ip;
-        const T ex = h[0], ez = h[g.np], es = h[2 * g.np];
-        const T *Bi = a.Bimg + g.origin;
-        const int cpl = c + 1 == nx ? 0 : c + 1, cmi = c == 0 ? nx - 1 : c - 1;
-        const int rpl = r + 1 == nz ? 0 : r + 1, rmi = r == 0 ? nz - 1 : r - 1;
-        const i64 qxp = (i64)r * g.ld + cpl, qxm = (i64)r * g.ld + cmi;
-        const i64 qzp = (i64)rpl * g.ld + c, qzm = (i64)rmi * g.ld + c;
-        const T i2dx = T(2) * g.dx, i2dz = T(2) * g.dz;
-        const T dxBU = (Bi[qxp] * Ua[qxp] - Bi[qxm] * Ua[qxm]) / i2dx;
-        const T dzBV = (Bi[qzp] * Va[qzp] - Bi[qzm] * Va[qzm]) / i2dz;
-        const T dzBU = (Bi[qzp] * Ua[qzp] - Bi[qzm] * Ua[qzm]) / i2dz;
-        const T dxBV = (Bi[qxp] * Va[qxp] - Bi[qxm] * Va[qxm]) / i2dx;
-        const T glam = (ex + ez) * (dxBU + dzBV);
-        const T gmu = T(2) * ex * dxBU + T(2) * ez * dzBV + es * (dzBU + dxBV);
-        if (a.gparams == 3) {
-            // density term: U_a(t) pairs with v(t-1) - v(t+1) (summation by
-            // parts of sum_{t=1}^{nt-2} v(t) (U_a(t+1) - U_a(t-1)) / 2dt)
-            T grl = 0;
-            const T i2dt = T(2) * g.dt;
-            if (it >= 2 && it <= sh.nt - 1) {
-                const T *h2 = a.hist2 + (hs + it - 1) * 2 * g.np + ip;
-                grl += Ua[q] * h2[0] / i2dt + Va[q] * h2[g.np] / i2dt;
+#pragma unroll
+        for (int k = 0; k < (NXH + 255) / 256; ++k) {
+            const int i = tid + 256 * k;
+            if (NXH % 256 == 0 || i < NXH) {
+                const int zi = i / (TILE_X + 8), xi = i - zi * (TILE_X + 8);
+                const int r = r0 + zi, c1 = c0 - 4 + xi;
+                T u = 0, v = 0;
+                if (r < nz && c1 >= 0 && c1 < nx) {
+                    u = S.DxBUa(r, c1);
+                    v = S.DxMesa(r, c1);
+                }
+                sxBU[zi][xi] = u;
+                sxMes[zi][xi] = v;
             }
-            if (it <= sh.nt - 3) {
-                const T *h2 = a.hist2 + (hs + it + 1) * 2 * g.np + ip;
-                grl -= Ua[q] * h2[0] / i2dt + Va[q] * h2[g.np] / i2dt;
+        }
+#pragma unroll
+        for (int k = 0; k < NZH / 256; ++k) {
+            const int i = tid + 256 * k;
+            const int zi = i / TILE_X, xi = i - zi * TILE_X;
+            const int r = r0 - 4 + zi, c1 = c0 + xi;
+            T u = 0, v = 0;
+            if (r >= 0 && r < nz && c1 < nx) {
+                v = S.DzBVa(r, c1);
+                u = S.DzMesa(r, c1);
             }
-            acc[0] += glam;
-            acc[g.np] += gmu;
-            acc[2 * g.np] += grl;
-        } else {
-            acc[0] += gmu - T(2) * glam;
-            if (a.gparams == 2) acc[g.np] += glam;
+            szBV[zi][xi] = v;
+            szMes[zi][xi] = u;
         }
     }
+    __syncthreads();
+    auto body = [&](auto tag) {
+        constexpr bool INT = decltype(tag)::value;
+#pragma unroll
+        for (int rr = 0; rr < RPT; ++rr) {
+            const int lz = threadIdx.y + BLOCK_Y * rr, r = r0 + lz;
+            if (!INT && (r >= nz || c >= nx)) continue;
+            const int xc = lx + 4, zc = lz + 4;
+            const i64 q = (i64)r * g.ld + c;
+            T Ax = 0, Az = 0, Mes = 0;
+#pragma unroll
+            for (int k = 1; k <= NK; ++k) {
+                const T ck = (T)fdc<FD>(k);
+                // (D+x)^T of Dx_(B U)_a (= L2M_ex_a = L_ez_a), rows [0, nz-2]
+                if (CND(r <= nz - 2)) {
+                    if (CND(c - k >= k - 1 && c - k <= nx - 1 - k)) Ax += ck * sxBU[lz][xc - k];
+                    if (CND(c + k - 1 >= k - 1 && c + k - 1 <= nx - 1 - k)) Ax += -ck * sxBU[lz][xc + k - 1];
+                }
+                // (D+z)^T of Dz_(B V)_a (= L2M_ez_a = L_ex_a), columns [1, nx-2]
+                if (CND(c >= 1 && c <= nx - 2)) {
+                    if (CND(r - k >= k - 1 && r - k <= nz - 1 - k)) Az += ck * szBV[zc - k][lx];
+                    if (CND(r + k - 1 >= k - 1 && r + k - 1 <= nz - 1 - k)) Az += -ck * szBV[zc + k - 1][lx];
+                }
+            }
+#pragma unroll
+            for (int k = 1; k <= NK; ++k) {
+                const T ck = (T)fdc<FD>(k);
+                // (D-x)^T of Dx_M_es_a, rows [0, nz-2]; term 1 cols [1, nx-2], k: [k, nx-k]
+                if (CND(r <= nz - 2)) {
+                    const int i1 = c - k + 1, i2 = c + k;
+                    if (CND(k == 1 ? (i1 >= 1 && i1 <= nx - 2) : (i1 >= k && i1 <= nx - k))) Mes += ck * sxMes[lz][xc - k + 1];
+                    if (CND(k == 1 ? (i2 >= 1 && i2 <= nx - 2) : (i2 >= k && i2 <= nx - k))) Mes += -ck * sxMes[lz][xc + k];
+                }
+            }
+#pragma unroll
+            for (int k = 1; k <= NK; ++k) {
+                const T ck = (T)fdc<FD>(k);
+                // (D-z)^T of Dz_M_es_a, columns [1, nx-2]; term 1 rows [1, nz-2], k: [k, nz-k]
+                if (CND(c >= 1 && c <= nx - 2)) {
+                    const int i1 = r - k + 1, i2 = r + k;
+                    if (CND(k == 1 ? (i1 >= 1 && i1 <= nz - 2) : (i1 >= k && i1 <= nz - k))) Mes += ck * szMes[zc - k + 1][lx];
+                    if (CND(k == 1 ? (i2 >= 1 && i2 <= nz - 2) : (i2 >= k && i2 <= nz - k))) Mes += -ck * szMes[zc + k][lx];
+                }
+            }
+            if (!INT && g.fs && r == 0 && c >= 1 && c <= nx - 2) Mes += T(2 * fdc<FD>(1)) * szMes[zc][lx];
+
+            const T l2m = a.L2M[g.origin + q], mu = a.M[g.origin + q];
+            const T lam = l2m - T(2) * mu;
+            const T exa0 = exa[q], eza0 = eza[q];
+            exa[q] = exa0 + l2m * Ax + lam * Az;
+            eza[q] = eza0 + lam * Ax + l2m * Az;
+            esa[q] = esa[q] + T(2) * mu * Mes;
+
+            if (!INT) {
+                const int ix = xstrip(g, r, c), iz = zstrip(g, r, c);
+                if (ix >= 0) {
+                    T *px = a.psix_n + s * a.psix_stride + ix;
+                    px[2 * a.psix_len] = pmlT_psi(S.GxMes(r, c), S.pxo[2 * a.psix_len + ix], a.profx, c, nx);
+                    px[3 * a.psix_len] = pmlT_psi(S.GxBU(r, c), S.pxo[3 * a.psix_len + ix], a.profx, c, nx);
+                }
+                if (iz >= 0) {
+                    T *pz = a.psiz_n + s * a.psiz_stride + iz;
+                    pz[2 * a.psiz_len] = pmlT_psi(S.GzBV(r, c), S.pzo[2 * a.psiz_len + iz], a.profz, r, nz);
+                    pz[3 * a.psiz_len] = pmlT_psi(S.GzMes(r, c), S.pzo[3 * a.psiz_len + iz], a.profz, r, nz);
+                }
+            }
+
+            if (a.snap) {
+                const T vals[5] = {Ua[q], Va[q], exa[q], eza[q], esa[q]};
+                int k = 0;
+                for (int f = 0; f < 5; ++f)
+                    if (a.snap_mask & (1 << f)) {
+                        a.snap[(((i64)s * a.nsnap + k) * a.nt_max + it) * nz * nx + (i64)r * nx + c] = vals[f];
+                        ++k;
+                    }
+            }
+
+            // imaging on the physical cells
+            const int rp = r - g.z0, cp = c - g.pad;
+            if (a.imaging == 0 || !CND(rp >= 0 && rp < g.nzd && cp >= 0 && cp < g.nxd)) continue;
+            const i64 ip = (i64)rp * g.nxd + cp;
+            T *acc = a.gacc + (i64)s * a.nacc * g.np + ip;
+            const i64 hs = (i64)s * a.nt_max;
+            if (a.imaging == 2) {
+                // exact discrete gradient: model dependence of step it, paired with
+                // its carry-in strains (history frame it-1)
+                T gl2m = 0, gl = 0, gm = 0, gb = 0;
+                if (it > 0) {
+                    const T *h = a.hist + (hs + it - 1) * 3 * g.np + ip;
+                    const T exm = h[0], ezm = h[g.np], esm = h[2 * g.np];
+                    gl2m = exm * Ax + ezm * Az;
+                    gl = ezm * Ax + exm * Az;
+                    gm = T(2) * esm * Mes;
+                }
+                if (!INT && g.fs && r == 0 && c <= nx - 2) {
+                    // sigma_zz = 0 row: Dz_V = L_ratio Dx_U dz/dx; row 0 has no z-PML
+                    const T tt = g.dtdz * eza0 * a.dxu0[(hs + it) * nx + c] * g.dz / g.dx;
+                    gl += -tt / l2m;
+                    gl2m += tt * lam / (l2m * l2m);
+                }
+                if (sh.type == 1 && r == sh.sz && c == sh.sx) {
+                    const T lm = lam + mu;
+                    const T stm = a.wav[hs + it] * g.dt / (T(2) * lm * g.dx * g.dz);
+                    const T Qm = -stm / lm * (exa0 + eza0);
+                    gl += Qm;
+                    gm += Qm;
+                }
+                if (a.gparams == 3) {
+                    const T *h2 = a.hist2 + (hs + it) * 2 * g.np + ip;
+                    gb = Ua[q] * h2[0] + Va[q] * h2[g.np];
+                    acc[0] += gl2m;
+                    acc[g.np] += gl;
+                    acc[2 * g.np] += gm;
+                    acc[3 * g.np] += gb;
+                } else {
+                    acc[0] += gm - T(2) * gl;
+                    if (a.gparams == 2) acc[g.np] += gl2m + gl;
+                }
+            } else {
+                // Zhou imaging (fwi/imaging.py:125-152): forward frame it with the
+                // adjoint carry-out of step it; centred periodic differences of B v_a
+                const T *h = a.hist + (hs + it) * 3 * g.np + ip;
+                const T ex = h[0], ez = h[g.np], es = h[2 * g.np];
+                const T *Bi = a.Bimg + g.origin;
+                const int cpl = c + 1 == nx ? 0 : c + 1, cmi = c == 0 ? nx - 1 : c - 1;
+                const int rpl = r + 1 == nz ? 0 : r + 1, rmi = r == 0 ? nz - 1 : r - 1;
+                const i64 qxp = (i64)r * g.ld + cpl, qxm = (i64)r * g.ld + cmi;
+                const i64 qzp = (i64)rpl * g.ld + c, qzm = (i64)rmi * g.ld + c;
+                const T i2dx = T(2) * g.dx, i2dz = T(2) * g.dz;
+                const T dxBU = (Bi[qxp] * Ua[qxp] - Bi[qxm] * Ua[qxm]) / i2dx;
+                const T dzBV = (Bi[qzp] * Va[qzp] - Bi[qzm] * Va[qzm]) / i2dz;
+                const T dzBU = (Bi[qzp] * Ua[qzp] - Bi[qzm] * Ua[qzm]) / i2dz;
+                const T dxBV = (Bi[qxp] * Va[qxp] - Bi[qxm] * Va[qxm]) / i2dx;
+                const T glam = (ex + ez) * (dxBU + dzBV);
+                const T gmu = T(2) * ex * dxBU + T(2) * ez * dzBV + es * (dzBU + dxBV);
+                if (a.gparams == 3) {
+                    // density term: U_a(t) pairs with v(t-1) - v(t+1) (summation by
+                    // parts of sum_{t=1}^{nt-2} v(t) (U_a(t+1) - U_a(t-1)) / 2dt)
+                    T grl = 0;
+                    const T i2dt = T(2) * g.dt;
+                    if (it >= 2 && it <= sh.nt - 1) {
+                        const T *h2 = a.hist2 + (hs + it - 1) * 2 * g.np + ip;
+                        grl += Ua[q] * h2[0] / i2dt + Va[q] * h2[g.np] / i2dt;
+                    }
+                    if (it <= sh.nt - 3) {
+                        const T *h2 = a.hist2 + (hs + it + 1) * 2 * g.np + ip;
+                        grl -= Ua[q] * h2[0] / i2dt + Va[q] * h2[g.np] / i2dt;
+                    }
+                    acc[0] += glam;
+                    acc[g.np] += gmu;
+                    acc[2 * g.np] += grl;
+                } else {
+                    acc[0] += gmu - T(2) * glam;
+                    if (a.gparams == 2) acc[g.np] += glam;
+                }
+            }
+        }
+    };
+    if (inter) body(std::true_type{});
+    else body(std::false_type{});
 }
+
 
 // sum of per-shot accumulators and chain rule to (Vs, Vp, rho)
 template <typename T, typename C>

@@ -1,0 +1,196 @@
+// Gradient post-processing and preconditioner kernels (gfx950).
+//   k_smooth_rows / k_smooth_cols  separable 'same'-mode zero-padded Gaussian
+//                                  (fwi/loss.py:17-30; the normalised 2-D
+//                                  kernel is the outer product of two
+//                                  normalised 1-D kernels)
+//   k_illum_hinv                   H = 1/(Ws + eps max Ws) / max(...)
+//                                  (fwi/illumination.py:38-48)
+//   k_eprecond3                    Plessix-Mulder weight
+//                                  (fwi/preconditioning.py:107-139)
+//   k_sos_traces                   DF-I biquad cascade + taper along t for
+//                                  independent traces (fwi/filters.py)
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace stfwi {
+
+#define MAX_SMOOTH_R 64
+struct SmoothW {
+    double w[2 * MAX_SMOOTH_R + 1];
+};
+
+template <typename T>
+__global__ void k_smooth_rows(const T *__restrict__ in, T *__restrict__ out, int nz, int nx, int r, SmoothW w)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x, z = blockIdx.y;
+    if (c >= nx || z >= nz) return;
+    double s = 0;
+    for (int k = -r; k <= r; ++k) {
+        const int cc = c + k;
+        if (cc >= 0 && cc < nx) s += w.w[k + r] * (double)in[(long long)z * nx + cc];
+    }
+    out[(long long)z * nx + c] = (T)s;
+}
+
+template <typename T>
+__global__ void k_smooth_cols(const T *__restrict__ in, T *__restrict__ out, int nz, int nx, int r, SmoothW w)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x, z = blockIdx.y;
+    if (c >= nx || z >= nz) return;
+    double s = 0;
+    for (int k = -r; k <= r; ++k) {
+        const int zz = z + k;
+        if (zz >= 0 && zz < nz) s += w.w[k + r] * (double)in[(long long)zz * nx + c];
+    }
+    out[(long long)z * nx + c] = (T)s;
+}
+
+// sum over shots of the padded energy, crop [0:nz_c, pad:pad+nx_c] (the
+// reference's crop assumes a free surface), then 1/(Ws + eps max)
+// normalised.  One block; grids here are at most a few 1e5 cells.
+template <typename T>
+__global__ void k_illum_hinv(const T *__restrict__ ws, int n_shots, int nz_t, int nx_t, int z0, int pad, int nz_c,
+                             int nx_c, double eps, T *out)
+{
+    __shared__ double red[256];
+    const long long N = (long long)nz_c * nx_c, NT = (long long)nz_t * nx_t;
+    double mx = 0;
+    for (long long i = threadIdx.x; i < N; i += blockDim.x) {
+        const long long r = i / nx_c, c = i % nx_c;
+        double s = 0;
+        for (int k = 0; k < n_shots; ++k) s += (double)ws[k * NT + (r + z0) * nx_t + c + pad];
+        out[i] = (T)s;
+        mx = fmax(mx, s);
+    }
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    const double lam = eps * red[0];
+    __syncthreads();
+    double hmax = 0;
+    for (long long i = threadIdx.x; i < N; i += blockDim.x) {
+        const double h = 1.0 / ((double)out[i] + lam);
+        out[i] = (T)h;
+        hmax = fmax(hmax, h);
+    }
+    red[threadIdx.x] = hmax;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    const double hm = red[0];
+    __syncthreads();
+    for (long long i = threadIdx.x; i < N; i += blockDim.x) out[i] = (T)((double)out[i] / hm);
+}
+
+// eprecond3 on the padded grid of the summed energy, then crop to the
+// physical window and normalise 1/We to max 1 (fwi/optim_lbfgsb.py:90-100).
+// One block.
+template <typename T>
+__global__ void k_eprecond3(const T *__restrict__ ws, int n_shots, int nz_t, int nx_t, int z0, int pad, int nzd,
+                            int nxd, int horizontal, double lmin, double lmax, double perp, double dx, double dz,
+                            double eps, T *we_pad, T *out)
+{
+    __shared__ double red[256];
+    const long long NT = (long long)nz_t * nx_t;
+    double mx = 0;
+    for (long long i = threadIdx.x; i < NT; i += blockDim.x) {
+        const int r = (int)(i / nx_t), c = (int)(i % nx_t);
+        double s = 0;
+        for (int k = 0; k < n_shots; ++k) s += (double)ws[k * NT + i];
+        const double cz = (double)(r - z0) * dz, cx = (double)(c - pad) * dx;
+        const double cperp = horizontal ? cz : cx, cline = horizontal ? cx : cz, sp = horizontal ? dz : dx;
+        const double y = fmax(fabs(cperp - perp), sp);
+        double we = sqrt(s) * (asinh((lmax - cline) / y) - asinh((lmin - cline) / y));
+        we = fabs(we);
+        we_pad[i] = (T)we;
+        mx = fmax(mx, we);
+    }
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    const double add = eps * red[0];
+    __syncthreads();
+    const long long N = (long long)nzd * nxd;
+    double pm = 0;
+    for (long long i = threadIdx.x; i < N; i += blockDim.x) {
+        const long long r = i / nxd, c = i % nxd;
+        const double p = 1.0 / ((double)we_pad[(r + z0) * nx_t + c + pad] + add);
+        out[i] = (T)p;
+        pm = fmax(pm, p);
+    }
+    red[threadIdx.x] = pm;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    const double m = red[0];
+    __syncthreads();
+    for (long long i = threadIdx.x; i < N; i += blockDim.x) out[i] = (T)((double)out[i] / m);
+}
+
+// independent traces x[t][trace]: taper the last n_taper samples then the
+// DF-I cascade (forward), or the exact transpose (cascade in reverse time,
+// sections in reverse order, then the taper)
+template <typename T>
+struct SosArgs {
+    T *x;
+    int nt, ntr, n_sos, n_taper, transpose;
+    T coef[8][5];
+};
+
+template <typename T>
+__global__ void k_sos_traces(SosArgs<T> a)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= a.ntr) return;
+    const int n = a.n_taper < a.nt ? a.n_taper : a.nt;
+    auto tw = [&](int t) -> T {
+        if (n <= 0 || t < a.nt - n) return T(1);
+        const int k = t - (a.nt - n) + 1;
+        return (T)((cos((double)k / n * M_PI) + 1.0) / 2.0);
+    };
+    if (!a.transpose) {
+        T x1[8], x2[8], y1[8], y2[8];
+        for (int k = 0; k < a.n_sos; ++k) x1[k] = x2[k] = y1[k] = y2[k] = 0;
+        for (int t = 0; t < a.nt; ++t) {
+            T v = a.x[(long long)t * a.ntr + j];
+            if (a.n_taper > 0) v = v * tw(t);
+            for (int k = 0; k < a.n_sos; ++k) {
+                const T *cf = a.coef[k];
+                const T y = cf[0] * v + cf[1] * x1[k] + cf[2] * x2[k] - cf[3] * y1[k] - cf[4] * y2[k];
+                x2[k] = x1[k]; x1[k] = v; y2[k] = y1[k]; y1[k] = y;
+                v = y;
+            }
+            a.x[(long long)t * a.ntr + j] = v;
+        }
+    } else {
+        T cx1[8], cx2[8], cy1[8], cy2[8];
+        for (int k = 0; k < a.n_sos; ++k) cx1[k] = cx2[k] = cy1[k] = cy2[k] = 0;
+        for (int t = a.nt - 1; t >= 0; --t) {
+            T gcur = a.x[(long long)t * a.ntr + j];
+            for (int k = a.n_sos - 1; k >= 0; --k) {
+                const T *cf = a.coef[k];
+                const T gy = gcur + cy1[k];
+                const T gx = cx1[k] + cf[0] * gy;
+                cx1[k] = cx2[k] + cf[1] * gy;
+                cx2[k] = cf[2] * gy;
+                cy1[k] = cy2[k] - cf[3] * gy;
+                cy2[k] = -cf[4] * gy;
+                gcur = gx;
+            }
+            if (a.n_taper > 0) gcur = gcur * tw(t);
+            a.x[(long long)t * a.ntr + j] = gcur;
+        }
+    }
+}
+
+}  // namespace stfwi

@@ -9,10 +9,12 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/stfwi.h"
 #include "fwi_kernels.h"
+#include "post_kernels.h"
 
 using namespace stfwi;
 
@@ -53,6 +55,7 @@ struct stfwi_plan {
     void *hist = nullptr, *hist2 = nullptr, *dxu0 = nullptr;
     void *gacc = nullptr;
     int hist2_mode = 0;
+    int tz[4] = {16, 16, 16, 16};   // tile height per time-step kernel (autotuned)
     ShotDev *shots_d = nullptr;
     i64 *rec_cell_d = nullptr, *uc_cell_d = nullptr;
     int *uc_ptr_d = nullptr, *uc_rec_d = nullptr;
@@ -83,6 +86,13 @@ static void fill(void *ptr, i64 n, hipStream_t st)
     if (n <= 0) return;
     int blocks = (int)std::min<i64>((n + 255) / 256, 65536);
     hipLaunchKernelGGL(k_fill<T>, dim3(blocks), dim3(256), 0, st, (T *)ptr, n, T(0));
+}
+
+static int popcount5(int m)
+{
+    int n = 0;
+    for (int f = 0; f < 5; ++f) n += (m >> f) & 1;
+    return n;
 }
 
 template <typename T>
@@ -131,6 +141,7 @@ extern "C" int stfwi_plan_create(const stfwi_config *cfg, stfwi_plan **out)
     stfwi_plan *p = new stfwi_plan();
     p->cfg = c;
     p->esz = c.dtype == STFWI_F64 ? 8 : 4;
+    for (int k = 0; k < 4; ++k) p->tz[k] = c.dtype == STFWI_F64 ? 8 : 16;
     p->nz = c.nz_dom + (c.free_surface ? c.pad : 2 * c.pad);
     p->nx = c.nx_dom + 2 * c.pad;
     if (p->nz < 10 || p->nx < 10) {
@@ -285,19 +296,66 @@ extern "C" int stfwi_set_shots(stfwi_plan *p, const stfwi_shot *shots, int n_sho
     return 0;
 }
 
-static int popcount5(int m)
+
+// tile heights compiled per precision: 8/16/32 rows in FP32, 8 in FP64
+// (larger FP64 tiles would exceed 64 KiB of LDS in the forward kernel)
+template <typename T>
+static bool tz_supported(int tz)
 {
-    int n = 0;
-    for (int f = 0; f < 5; ++f) n += (m >> f) & 1;
-    return n;
+    return std::is_same<T, float>::value ? (tz == 8 || tz == 16 || tz == 32) : tz == 8;
+}
+
+// kernel ids: 0 forward velocity, 1 forward strain, 2 adjoint strain,
+// 3 adjoint velocity
+template <typename T, int FD>
+static void launch_fwd(int which, int tz, FwdArgs<T> a, int rblk, int S, hipStream_t st, int it)
+{
+    a.tiles_z = (a.g.nz + tz - 1) / tz;
+    a.ntiles = a.tiles_x * a.tiles_z;
+    dim3 grid(a.ntiles + rblk, 1, S), blk(TILE_X, BLOCK_Y);
+#define STFWI_LF(TZV)                                                                         \
+    do {                                                                                      \
+        if (which == 0) hipLaunchKernelGGL((k_fwd_vel<T, FD, TZV>), grid, blk, 0, st, a, it);    \
+        else hipLaunchKernelGGL((k_fwd_strain<T, FD, TZV>), grid, blk, 0, st, a, it);            \
+    } while (0)
+    if constexpr (std::is_same<T, float>::value) {
+        if (tz == 32) STFWI_LF(32);
+        else if (tz == 16) STFWI_LF(16);
+        else STFWI_LF(8);
+    } else {
+        STFWI_LF(8);
+    }
+#undef STFWI_LF
 }
 
 template <typename T, int FD>
-static int forward_t(stfwi_plan *p, void *rec, int rec_mask, void *illum, int illum_comp, void *snap, int snap_mask,
-                     hipStream_t st)
+static void launch_adj(int which, int tz, AdjArgs<T> a, int S, hipStream_t st, int it)
+{
+    a.tiles_z = (a.g.nz + tz - 1) / tz;
+    a.ntiles = a.tiles_x * a.tiles_z;
+    dim3 grid(a.ntiles, 1, S), blk(TILE_X, BLOCK_Y);
+#define STFWI_LA(TZV)                                                                          \
+    do {                                                                                       \
+        if (which == 2) hipLaunchKernelGGL((k_adj_strain<T, FD, TZV>), grid, blk, 0, st, a, it);  \
+        else hipLaunchKernelGGL((k_adj_vel<T, FD, TZV>), grid, blk, 0, st, a, it);                \
+    } while (0)
+    if constexpr (std::is_same<T, float>::value) {
+        if (tz == 32) STFWI_LA(32);
+        else if (tz == 16) STFWI_LA(16);
+        else STFWI_LA(8);
+    } else {
+        STFWI_LA(8);
+    }
+#undef STFWI_LA
+}
+
+template <typename T>
+static FwdArgs<T> fwd_args(stfwi_plan *p, void *rec, int rec_mask, void *illum, int illum_comp, void *snap,
+                           int snap_mask)
 {
     const stfwi_config &c = p->cfg;
     FwdArgs<T> a;
+    memset(&a, 0, sizeof a);
     a.g = make_geo<T>(p);
     a.shots = p->shots_d;
     const T *mod = (const T *)p->mod;
@@ -315,22 +373,54 @@ static int forward_t(stfwi_plan *p, void *rec, int rec_mask, void *illum, int il
     a.dxu0 = (T *)p->dxu0;
     a.illum = (T *)illum; a.illum_comp = illum_comp;
     a.snap = (T *)snap; a.snap_mask = snap_mask; a.nsnap = popcount5(snap_mask);
-    a.tiles_x = (p->nx + FWD_TILE_X - 1) / FWD_TILE_X;
-    a.tiles_z = (p->nz + FWD_TILE_Z - 1) / FWD_TILE_Z;
-    a.ntiles = a.tiles_x * a.tiles_z;
+    a.tiles_x = (p->nx + TILE_X - 1) / TILE_X;
+    return a;
+}
+
+template <typename T>
+static AdjArgs<T> adj_args(stfwi_plan *p, const void *rec_adj, void *src_adj, void *snap, int snap_mask)
+{
+    const stfwi_config &c = p->cfg;
+    AdjArgs<T> a;
+    memset(&a, 0, sizeof a);
+    a.g = make_geo<T>(p);
+    a.shots = p->shots_d;
+    const T *mod = (const T *)p->mod;
+    a.L2M = mod; a.M = mod + p->fstride; a.B = mod + 2 * p->fstride; a.Bimg = mod + 3 * p->fstride;
+    a.Lr0 = mod + 4 * p->fstride;
+    a.state = (T *)p->state; a.sstride = 5 * p->fstride;
+    a.psix_len = p->psix_len; a.psiz_len = p->psiz_len;
+    a.psix_stride = 4 * p->psix_len; a.psiz_stride = 4 * p->psiz_len;
+    a.profx = (const T *)p->profx; a.profz = (const T *)p->profz;
+    a.wav = (const T *)p->wav; a.nt_max = c.nt_max;
+    a.rec_adj = (const T *)rec_adj; a.nrec_max = c.nrec_max;
+    a.uc_cell = p->uc_cell_d; a.uc_ptr = p->uc_ptr_d; a.uc_rec = p->uc_rec_d;
+    a.src_adj = (T *)src_adj;
+    a.imaging = p->hist ? c.imaging : 0;
+    a.gparams = c.grad_params; a.nacc = p->nacc;
+    a.hist = (const T *)p->hist; a.hist2 = (const T *)p->hist2; a.dxu0 = (const T *)p->dxu0;
+    a.gacc = (T *)p->gacc;
+    a.snap = (T *)snap; a.snap_mask = snap_mask; a.nsnap = popcount5(snap_mask);
+    a.tiles_x = (p->nx + TILE_X - 1) / TILE_X;
+    return a;
+}
+
+template <typename T, int FD>
+static int forward_t(stfwi_plan *p, void *rec, int rec_mask, void *illum, int illum_comp, void *snap, int snap_mask,
+                     hipStream_t st)
+{
+    FwdArgs<T> a = fwd_args<T>(p, rec, rec_mask, illum, illum_comp, snap, snap_mask);
     const int S = p->n_shots;
     fill<T>(p->state, (i64)S * 5 * p->fstride, st);
     fill<T>(p->psix_f, (i64)S * 5 * p->psix_len, st);
     fill<T>(p->psiz_f, (i64)S * 5 * p->psiz_len, st);
     const int rblk = rec ? (p->max_nrec + 255) / 256 : 0;
-    dim3 blk(FWD_TILE_X, FWD_TILE_Z);
-    dim3 gv(a.ntiles + rblk, 1, S), gs(a.ntiles + rblk, 1, S);
     for (int it = 0; it < p->nt_run; ++it) {
-        hipLaunchKernelGGL((k_fwd_vel<T, FD>), gv, blk, 0, st, a, it);
-        hipLaunchKernelGGL((k_fwd_strain<T, FD>), gs, blk, 0, st, a, it);
+        launch_fwd<T, FD>(0, p->tz[0], a, rblk, S, st, it);
+        launch_fwd<T, FD>(1, p->tz[1], a, rblk, S, st, it);
     }
     // exx/ezz receivers of the last step
-    if (rec) hipLaunchKernelGGL((k_fwd_vel<T, FD>), gv, blk, 0, st, a, p->nt_run);
+    if (rec) launch_fwd<T, FD>(0, p->tz[0], a, rblk, S, st, p->nt_run);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -360,29 +450,7 @@ extern "C" int stfwi_forward(stfwi_plan *p, void *rec, int rec_mask, void *illum
 template <typename T, int FD>
 static int adjoint_t(stfwi_plan *p, const void *rec_adj, void *src_adj, void *snap, int snap_mask, hipStream_t st)
 {
-    const stfwi_config &c = p->cfg;
-    AdjArgs<T> a;
-    a.g = make_geo<T>(p);
-    a.shots = p->shots_d;
-    const T *mod = (const T *)p->mod;
-    a.L2M = mod; a.M = mod + p->fstride; a.B = mod + 2 * p->fstride; a.Bimg = mod + 3 * p->fstride;
-    a.Lr0 = mod + 4 * p->fstride;
-    a.state = (T *)p->state; a.sstride = 5 * p->fstride;
-    a.psix_len = p->psix_len; a.psiz_len = p->psiz_len;
-    a.psix_stride = 4 * p->psix_len; a.psiz_stride = 4 * p->psiz_len;
-    a.profx = (const T *)p->profx; a.profz = (const T *)p->profz;
-    a.wav = (const T *)p->wav; a.nt_max = c.nt_max;
-    a.rec_adj = (const T *)rec_adj; a.nrec_max = c.nrec_max;
-    a.uc_cell = p->uc_cell_d; a.uc_ptr = p->uc_ptr_d; a.uc_rec = p->uc_rec_d;
-    a.src_adj = (T *)src_adj;
-    a.imaging = p->hist ? c.imaging : 0;
-    a.gparams = c.grad_params; a.nacc = p->nacc;
-    a.hist = (const T *)p->hist; a.hist2 = (const T *)p->hist2; a.dxu0 = (const T *)p->dxu0;
-    a.gacc = (T *)p->gacc;
-    a.snap = (T *)snap; a.snap_mask = snap_mask; a.nsnap = popcount5(snap_mask);
-    a.tiles_x = (p->nx + FWD_TILE_X - 1) / FWD_TILE_X;
-    a.tiles_z = (p->nz + FWD_TILE_Z - 1) / FWD_TILE_Z;
-    a.ntiles = a.tiles_x * a.tiles_z;
+    AdjArgs<T> a = adj_args<T>(p, rec_adj, src_adj, snap, snap_mask);
     const int S = p->n_shots;
     fill<T>(p->state, (i64)S * 5 * p->fstride, st);
     for (int k = 0; k < 2; ++k) {
@@ -390,15 +458,14 @@ static int adjoint_t(stfwi_plan *p, const void *rec_adj, void *src_adj, void *sn
         fill<T>(p->psiz_a[k], (i64)S * 4 * p->psiz_len, st);
     }
     if (a.imaging) fill<T>(p->gacc, (i64)S * p->nacc * p->np, st);
-    dim3 blk(FWD_TILE_X, FWD_TILE_Z), grid(a.ntiles, 1, S);
     dim3 gi((p->max_nuc + 63) / 64, 1, S);
     for (int it = p->nt_run - 1; it >= 0; --it) {
         const int cur = it & 1, prev = cur ^ 1;
         a.psix_o = (const T *)p->psix_a[prev]; a.psiz_o = (const T *)p->psiz_a[prev];
         a.psix_n = (T *)p->psix_a[cur]; a.psiz_n = (T *)p->psiz_a[cur];
         hipLaunchKernelGGL(k_adj_inject<T>, gi, dim3(64), 0, st, a, it);
-        hipLaunchKernelGGL((k_adj_strain<T, FD>), grid, blk, 0, st, a, it);
-        hipLaunchKernelGGL((k_adj_vel<T, FD>), grid, blk, 0, st, a, it);
+        launch_adj<T, FD>(2, p->tz[2], a, S, st, it);
+        launch_adj<T, FD>(3, p->tz[3], a, S, st, it);
     }
     HIP_TRY(hipGetLastError());
     return 0;
@@ -493,66 +560,23 @@ extern "C" int stfwi_residual(stfwi_plan *p, const void *pred, const void *obs, 
 // the plan at a mid-record step, bracketed by HIP events on `stream`
 // ---------------------------------------------------------------------------
 template <typename T, int FD>
-static int time_kernel_t(stfwi_plan *p, int which, int reps, hipStream_t st, double *ms)
+static int time_kernel_t(stfwi_plan *p, int which, int tz, int reps, hipStream_t st, double *ms)
 {
-    const stfwi_config &c = p->cfg;
     const int it = p->nt_run / 2;
-    dim3 blk(FWD_TILE_X, FWD_TILE_Z);
-    const int tiles_x = (p->nx + FWD_TILE_X - 1) / FWD_TILE_X, tiles_z = (p->nz + FWD_TILE_Z - 1) / FWD_TILE_Z;
-    const int ntiles = tiles_x * tiles_z;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     if (which <= 1) {
-        FwdArgs<T> a;
-        memset(&a, 0, sizeof a);
-        a.g = make_geo<T>(p);
-        a.shots = p->shots_d;
-        const T *mod = (const T *)p->mod;
-        a.L2M = mod; a.M = mod + p->fstride; a.B = mod + 2 * p->fstride; a.Lr0 = mod + 4 * p->fstride;
-        a.state = (T *)p->state; a.sstride = 5 * p->fstride;
-        a.psix = (T *)p->psix_f; a.psiz = (T *)p->psiz_f;
-        a.psix_n = p->psix_len; a.psiz_n = p->psiz_len;
-        a.psix_stride = 5 * p->psix_len; a.psiz_stride = 5 * p->psiz_len;
-        a.profx = (const T *)p->profx; a.profz = (const T *)p->profz;
-        a.wav = (const T *)p->wav; a.nt_max = c.nt_max; a.nrec_max = c.nrec_max;
-        a.rec_cell = p->rec_cell_d;
-        a.hist = (T *)p->hist; a.hist2 = (T *)p->hist2; a.hist2_mode = p->hist ? p->hist2_mode : 0;
-        a.dxu0 = (T *)p->dxu0;
-        a.tiles_x = tiles_x; a.tiles_z = tiles_z; a.ntiles = ntiles;
-        dim3 grid(ntiles, 1, p->n_shots);
+        FwdArgs<T> a = fwd_args<T>(p, nullptr, 0, nullptr, 0, nullptr, 0);
         HIP_TRY(hipEventRecord(e0, st));
-        for (int r = 0; r < reps; ++r) {
-            if (which == 0) hipLaunchKernelGGL((k_fwd_vel<T, FD>), grid, blk, 0, st, a, it);
-            else hipLaunchKernelGGL((k_fwd_strain<T, FD>), grid, blk, 0, st, a, it);
-        }
+        for (int r = 0; r < reps; ++r) launch_fwd<T, FD>(which, tz, a, 0, p->n_shots, st, it);
         HIP_TRY(hipEventRecord(e1, st));
     } else {
-        AdjArgs<T> a;
-        memset(&a, 0, sizeof a);
-        a.g = make_geo<T>(p);
-        a.shots = p->shots_d;
-        const T *mod = (const T *)p->mod;
-        a.L2M = mod; a.M = mod + p->fstride; a.B = mod + 2 * p->fstride; a.Bimg = mod + 3 * p->fstride;
-        a.Lr0 = mod + 4 * p->fstride;
-        a.state = (T *)p->state; a.sstride = 5 * p->fstride;
-        a.psix_len = p->psix_len; a.psiz_len = p->psiz_len;
-        a.psix_stride = 4 * p->psix_len; a.psiz_stride = 4 * p->psiz_len;
+        AdjArgs<T> a = adj_args<T>(p, nullptr, nullptr, nullptr, 0);
         a.psix_o = (const T *)p->psix_a[0]; a.psiz_o = (const T *)p->psiz_a[0];
         a.psix_n = (T *)p->psix_a[1]; a.psiz_n = (T *)p->psiz_a[1];
-        a.profx = (const T *)p->profx; a.profz = (const T *)p->profz;
-        a.wav = (const T *)p->wav; a.nt_max = c.nt_max; a.nrec_max = c.nrec_max;
-        a.imaging = p->hist ? c.imaging : 0;
-        a.gparams = c.grad_params; a.nacc = p->nacc;
-        a.hist = (const T *)p->hist; a.hist2 = (const T *)p->hist2; a.dxu0 = (const T *)p->dxu0;
-        a.gacc = (T *)p->gacc;
-        a.tiles_x = tiles_x; a.tiles_z = tiles_z; a.ntiles = ntiles;
-        dim3 grid(ntiles, 1, p->n_shots);
         HIP_TRY(hipEventRecord(e0, st));
-        for (int r = 0; r < reps; ++r) {
-            if (which == 2) hipLaunchKernelGGL((k_adj_strain<T, FD>), grid, blk, 0, st, a, it);
-            else hipLaunchKernelGGL((k_adj_vel<T, FD>), grid, blk, 0, st, a, it);
-        }
+        for (int r = 0; r < reps; ++r) launch_adj<T, FD>(which, tz, a, p->n_shots, st, it);
         HIP_TRY(hipEventRecord(e1, st));
     }
     HIP_TRY(hipGetLastError());
@@ -565,16 +589,169 @@ static int time_kernel_t(stfwi_plan *p, int which, int reps, hipStream_t st, dou
     return 0;
 }
 
+static int time_kernel(stfwi_plan *p, int which, int tz, int reps, hipStream_t st, double *ms)
+{
+    const int fd = p->cfg.fd_order;
+    if (p->cfg.dtype == STFWI_F64)
+        return fd == 8 ? time_kernel_t<double, 8>(p, which, tz, reps, st, ms)
+             : fd == 4 ? time_kernel_t<double, 4>(p, which, tz, reps, st, ms)
+                       : time_kernel_t<double, 2>(p, which, tz, reps, st, ms);
+    return fd == 8 ? time_kernel_t<float, 8>(p, which, tz, reps, st, ms)
+         : fd == 4 ? time_kernel_t<float, 4>(p, which, tz, reps, st, ms)
+                   : time_kernel_t<float, 2>(p, which, tz, reps, st, ms);
+}
+
 extern "C" int stfwi_time_kernel(stfwi_plan *p, int which, int reps, void *stream, double *ms)
 {
     if (!p || !ms || which < 0 || which > 3 || reps < 1) return fail(STFWI_ERR_INVALID, "bad argument");
     if (!p->have_model || !p->n_shots) return fail(STFWI_ERR_STATE, "model or shots not set");
-    if (which == 3 && p->cfg.imaging && !p->fwd_done) return fail(STFWI_ERR_STATE, "needs a forward first");
+    return time_kernel(p, which, p->tz[which], reps, (hipStream_t)stream, ms);
+}
+
+extern "C" int stfwi_autotune(stfwi_plan *p, int reps, void *stream, int *tz_out, double *us_out)
+{
+    if (!p || reps < 1) return fail(STFWI_ERR_INVALID, "bad argument");
+    if (!p->have_model || !p->n_shots) return fail(STFWI_ERR_STATE, "model or shots not set");
+    const int cand[3] = {8, 16, 32};
+    const bool f64 = p->cfg.dtype == STFWI_F64;
+    for (int which = 0; which < 4; ++which) {
+        double best = 1e300;
+        int best_tz = p->tz[which];
+        for (int ci = 0; ci < 3; ++ci) {
+            const int tz = cand[ci];
+            if (f64 ? !tz_supported<double>(tz) : !tz_supported<float>(tz)) continue;
+            double ms = 0;
+            int rc = time_kernel(p, which, tz, 2, (hipStream_t)stream, &ms);   // warm-up
+            if (!rc) rc = time_kernel(p, which, tz, reps, (hipStream_t)stream, &ms);
+            if (rc) return rc;
+            if (us_out) us_out[which * 3 + ci] = ms * 1e3 / reps;
+            if (ms < best) {
+                best = ms;
+                best_tz = tz;
+            }
+        }
+        p->tz[which] = best_tz;
+        if (tz_out) tz_out[which] = best_tz;
+    }
+    // the timing launches ran on the live buffers
+    p->fwd_done = false;
+    p->adj_done = false;
+    return 0;
+}
+
+extern "C" int stfwi_set_tiles(stfwi_plan *p, const int *tz)
+{
+    if (!p || !tz) return fail(STFWI_ERR_INVALID, "bad argument");
+    for (int k = 0; k < 4; ++k) {
+        const bool ok = p->cfg.dtype == STFWI_F64 ? tz_supported<double>(tz[k]) : tz_supported<float>(tz[k]);
+        if (!ok) return fail(STFWI_ERR_UNSUPPORTED, "tile height %d not compiled for this precision", tz[k]);
+    }
+    for (int k = 0; k < 4; ++k) p->tz[k] = tz[k];
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// plan-free post-processing entry points
+// ---------------------------------------------------------------------------
+template <typename T>
+static int smooth_t(const void *in, void *tmp, void *out, int nz, int nx, double sigma, hipStream_t st)
+{
+    const int r = (int)(3.0 * sigma + 0.5);
+    if (r > MAX_SMOOTH_R) return fail(STFWI_ERR_INVALID, "sigma=%g: kernel radius %d > %d", sigma, r, MAX_SMOOTH_R);
+    SmoothW w;
+    double sum = 0;
+    for (int k = -r; k <= r; ++k) {
+        const double x = k / sigma;
+        w.w[k + r] = exp(-0.5 * x * x);
+        sum += w.w[k + r];
+    }
+    for (int k = 0; k <= 2 * r; ++k) w.w[k] /= sum;
+    dim3 grid((nx + 127) / 128, nz);
+    hipLaunchKernelGGL(k_smooth_rows<T>, grid, dim3(128), 0, st, (const T *)in, (T *)tmp, nz, nx, r, w);
+    hipLaunchKernelGGL(k_smooth_cols<T>, grid, dim3(128), 0, st, (const T *)tmp, (T *)out, nz, nx, r, w);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int stfwi_smooth2d(const void *in, void *tmp, void *out, int nz, int nx, double sigma, int dtype,
+                              void *stream)
+{
+    if (!in || !tmp || !out || nz < 1 || nx < 1 || !(sigma > 0)) return fail(STFWI_ERR_INVALID, "bad argument");
+    if (tmp == in || tmp == out) return fail(STFWI_ERR_INVALID, "tmp must not alias in/out");
+    return dtype == STFWI_F64 ? smooth_t<double>(in, tmp, out, nz, nx, sigma, (hipStream_t)stream)
+                              : smooth_t<float>(in, tmp, out, nz, nx, sigma, (hipStream_t)stream);
+}
+
+extern "C" int stfwi_illumination(const void *ws, int n_shots, int nz_t, int nx_t, int z0, int pad, int nz_c,
+                                  int nx_c, double eps, int dtype, void *out, void *stream)
+{
+    if (!ws || !out || n_shots < 1 || nz_c < 1 || nx_c < 1 || z0 + nz_c > nz_t || pad + nx_c > nx_t)
+        return fail(STFWI_ERR_INVALID, "bad argument");
     hipStream_t st = (hipStream_t)stream;
-    const int fd = p->cfg.fd_order;
-    if (p->cfg.dtype == STFWI_F64)
-        return fd == 8 ? time_kernel_t<double, 8>(p, which, reps, st, ms)
-             : fd == 4 ? time_kernel_t<double, 4>(p, which, reps, st, ms) : time_kernel_t<double, 2>(p, which, reps, st, ms);
-    return fd == 8 ? time_kernel_t<float, 8>(p, which, reps, st, ms)
-         : fd == 4 ? time_kernel_t<float, 4>(p, which, reps, st, ms) : time_kernel_t<float, 2>(p, which, reps, st, ms);
+    if (dtype == STFWI_F64)
+        hipLaunchKernelGGL(k_illum_hinv<double>, dim3(1), dim3(256), 0, st, (const double *)ws, n_shots, nz_t, nx_t,
+                           z0, pad, nz_c, nx_c, eps, (double *)out);
+    else
+        hipLaunchKernelGGL(k_illum_hinv<float>, dim3(1), dim3(256), 0, st, (const float *)ws, n_shots, nz_t, nx_t,
+                           z0, pad, nz_c, nx_c, eps, (float *)out);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int stfwi_eprecond3(const void *ws, int n_shots, int nz_t, int nx_t, int free_surface, int pad, int nz_dom,
+                               int nx_dom, const int *rec_x, const int *rec_z, int nrec, double dx, double dz, double eps,
+                               int dtype, void *we_pad, void *out, void *stream)
+{
+    if (!ws || !we_pad || !out || !rec_x || !rec_z || nrec < 1 || n_shots < 1)
+        return fail(STFWI_ERR_INVALID, "bad argument");
+    // orientation of the receiver line: the coordinate with the larger spread
+    double mx = 0, mz = 0;
+    for (int i = 0; i < nrec; ++i) { mx += rec_x[i]; mz += rec_z[i]; }
+    mx /= nrec; mz /= nrec;
+    double vx = 0, vz = 0;
+    for (int i = 0; i < nrec; ++i) {
+        vx += (rec_x[i] - mx) * (rec_x[i] - mx);
+        vz += (rec_z[i] - mz) * (rec_z[i] - mz);
+    }
+    const int horiz = vx >= vz;
+    const int *ln = horiz ? rec_x : rec_z;
+    int lo = ln[0], hi = ln[0];
+    for (int i = 1; i < nrec; ++i) { lo = std::min(lo, ln[i]); hi = std::max(hi, ln[i]); }
+    const double s_line = horiz ? dx : dz, s_perp = horiz ? dz : dx;
+    const double lmin = lo * s_line, lmax = hi * s_line, perp = (horiz ? rec_z[0] : rec_x[0]) * s_perp;
+    const int z0 = free_surface ? 0 : pad;
+    hipStream_t st = (hipStream_t)stream;
+    if (dtype == STFWI_F64)
+        hipLaunchKernelGGL(k_eprecond3<double>, dim3(1), dim3(256), 0, st, (const double *)ws, n_shots, nz_t, nx_t, z0,
+                           pad, nz_dom, nx_dom, horiz, lmin, lmax, perp, dx, dz, eps, (double *)we_pad, (double *)out);
+    else
+        hipLaunchKernelGGL(k_eprecond3<float>, dim3(1), dim3(256), 0, st, (const float *)ws, n_shots, nz_t, nx_t, z0,
+                           pad, nz_dom, nx_dom, horiz, lmin, lmax, perp, dx, dz, eps, (float *)we_pad, (float *)out);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+template <typename T>
+static int sos_t(void *x, int nt, int ntr, const double *sos, int n_sos, int n_taper, int transpose, hipStream_t st)
+{
+    SosArgs<T> a;
+    a.x = (T *)x; a.nt = nt; a.ntr = ntr; a.n_sos = n_sos; a.n_taper = n_taper; a.transpose = transpose;
+    for (int k = 0; k < n_sos; ++k) {
+        const double *s = sos + 6 * k;
+        a.coef[k][0] = (T)(s[0] / s[3]); a.coef[k][1] = (T)(s[1] / s[3]); a.coef[k][2] = (T)(s[2] / s[3]);
+        a.coef[k][3] = (T)(s[4] / s[3]); a.coef[k][4] = (T)(s[5] / s[3]);
+    }
+    hipLaunchKernelGGL(k_sos_traces<T>, dim3((ntr + 127) / 128), dim3(128), 0, st, a);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int stfwi_sosfilt(void *x, int nt, int ntr, const double *sos, int n_sos, int n_taper, int transpose,
+                             int dtype, void *stream)
+{
+    if (!x || nt < 1 || ntr < 1 || n_sos < 0 || n_sos > 8 || (n_sos && !sos)) return fail(STFWI_ERR_INVALID, "bad argument");
+    for (int k = 0; k < n_sos; ++k)
+        if (sos[6 * k + 3] == 0.0) return fail(STFWI_ERR_INVALID, "sos section %d has a0 = 0", k);
+    return dtype == STFWI_F64 ? sos_t<double>(x, nt, ntr, sos, n_sos, n_taper, transpose, (hipStream_t)stream)
+                              : sos_t<float>(x, nt, ntr, sos, n_sos, n_taper, transpose, (hipStream_t)stream);
 }

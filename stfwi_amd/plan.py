@@ -213,6 +213,30 @@ class Plan:
         check(lib().stfwi_adjoint(self._h, _ptr(ra), _ptr(sa), _ptr(sn), smask, _stream()))
         return out
 
+    def autotune(self, reps=20):
+        """Pick the tile height of each time-step kernel by timing every
+        compiled variant on this plan's shots (run before forward()).
+        Returns (tile heights, {kernel: {tz: microseconds}})."""
+        tz = (C.c_int * 4)()
+        us = (C.c_double * 12)()
+        check(lib().stfwi_autotune(self._h, int(reps), _stream(), tz, us))
+        names = ("fwd_vel", "fwd_strain", "adj_strain", "adj_vel")
+        table = {n: {h: us[3 * i + j] for j, h in enumerate((8, 16, 32)) if us[3 * i + j] > 0}
+                 for i, n in enumerate(names)}
+        return list(tz), table
+
+    def set_tiles(self, tz):
+        arr = (C.c_int * 4)(*[int(t) for t in tz])
+        check(lib().stfwi_set_tiles(self._h, arr))
+
+    def time_kernel(self, which, reps=100):
+        """Average launch time in microseconds of one time-step kernel over
+        all shots (0 forward velocity, 1 forward strain, 2 adjoint strain,
+        3 adjoint velocity), measured with HIP events on the current stream."""
+        ms = C.c_double()
+        check(lib().stfwi_time_kernel(self._h, int(which), int(reps), _stream(), C.byref(ms)))
+        return ms.value * 1e3 / reps
+
     def finalize(self, vs, vp, rho):
         """(3, nz_dom, nx_dom) gradients w.r.t. (Vs, Vp, rho); the chain
         factors are formed in the dtype of the given model tensors."""

@@ -112,16 +112,16 @@ def test_gpu_batched_shots_equal_single():
     assert rel_l2(gb.cpu().numpy(), gsum.cpu().numpy()) < 1e-6
 
 
-@pytest.mark.parametrize("fd", [4, 8])
-def test_gpu_f32_gradient_close_to_f64_oracle(fd):
-    """FP32 HIP path against the FP64 oracle on a 1000-step case."""
+def _f32_case(fd, pert):
+    """FP32 HIP gradient and the FP64 / FP32 oracle gradients of one moment
+    shot (400 steps) whose observed data come from Vs * pert."""
     m = dp_model()
     m["nt"] = 400
     fs, st = True, "moment"
     sx, sz, rx, rz = dp_geometry(m["nz"], m["nx"], fs, st)
     wav = osig.ricker(np.arange(m["nt"]) * m["dt"], m["fc"], 1.5 / m["fc"])
     g = oracle.Geom(m["nz"], m["nx"], m["pad"], fs, fd, m["nt"], m["dx"], m["dz"], m["dt"], m["fc"])
-    obs, _ = oracle.forward(g, m["Vs"] * 1.02, m["Vp"], m["rho"], sx, sz, wav, rx, rz, st)
+    obs, _ = oracle.forward(g, m["Vs"] * pert, m["Vp"], m["rho"], sx, sz, wav, rx, rz, st)
     obs = obs[2]
     scale = 1.0 / np.mean(obs ** 2)
     p = _plan(m, fd, fs, torch.float32, nrec=rx.size, imaging="exact", grad="vs", hist=True)
@@ -131,12 +131,105 @@ def test_gpu_f32_gradient_close_to_f64_oracle(fd):
     loss, radj = p.residual(rec, torch.as_tensor(obs[None]), 2, None, 0, scale, 1)
     p.adjoint(radj)
     gg = p.finalize(m["Vs"], m["Vp"], m["rho"]).cpu().numpy()
-    rec_o, t = oracle.forward(g, m["Vs"], m["Vp"], m["rho"], sx, sz, wav, rx, rz, st, np.float64, ("e", "dxu0"))
-    l_o, cot = osig.residual_prep(rec_o[2], obs, None, 0, scale, 1)
-    ra = np.zeros_like(rec_o)
-    ra[2] = cot
-    a = oracle.adjoint(g, m["Vs"], m["Vp"], m["rho"], sx, sz, ra, rx, rz, st, wav, "exact", t)
-    go = oracle.exact_chain(g, a["g"], m["Vs"], m["Vp"], m["rho"])
-    assert rel_l2(rec[0].cpu().numpy(), rec_o) < TOL_F32
-    assert abs(float(loss[0]) / l_o - 1) < TOL_F32
-    assert rel_l2(gg[0], go[0]) < TOL_F32
+    out = {"gpu": (rec[0].cpu().numpy(), float(loss[0]), gg[0])}
+    for name, dt in (("o64", np.float64), ("o32", np.float32)):
+        rec_o, t = oracle.forward(g, m["Vs"], m["Vp"], m["rho"], sx, sz, wav, rx, rz, st, dt, ("e", "dxu0"))
+        l_o, cot = osig.residual_prep(rec_o[2].astype(np.float64), obs, None, 0, scale, 1)
+        ra = np.zeros((4,) + cot.shape)
+        ra[2] = cot
+        a = oracle.adjoint(g, m["Vs"], m["Vp"], m["rho"], sx, sz, ra, rx, rz, st, wav, "exact", t, None, dt)
+        go = oracle.exact_chain(g, a["g"].astype(np.float64), m["Vs"], m["Vp"], m["rho"])[0]
+        out[name] = (rec_o, l_o, go)
+    return out
+
+
+@pytest.mark.parametrize("fd", [4, 8])
+def test_gpu_f32_gradient_close_to_f64_oracle(fd):
+    """FP32 HIP path against the FP64 oracle: seismograms, misfit and the
+    gradient within 1e-5 relative L2 for a 10% Vs misfit."""
+    o = _f32_case(fd, 1.10)
+    rec, loss, g = o["gpu"]
+    assert rel_l2(rec, o["o64"][0]) < TOL_F32
+    assert abs(loss / o["o64"][1] - 1) < TOL_F32
+    assert rel_l2(g, o["o64"][2]) < TOL_F32
+
+
+@pytest.mark.parametrize("fd", [4, 8])
+def test_gpu_f32_ill_conditioned_no_worse_than_reference_algorithm(fd):
+    """With a 2% misfit the residual pred - obs cancels ~98% of the data, so
+    FP32 wavefield rounding is amplified ~50x in the gradient; the
+    reference's algorithm run in FP32 (the oracle's float build) is then
+    ~2-3e-5 from FP64.  The HIP FP32 gradient must be no worse than that."""
+    o = _f32_case(fd, 1.02)
+    e_gpu = rel_l2(o["gpu"][2], o["o64"][2])
+    e_ref32 = rel_l2(o["o32"][2], o["o64"][2])
+    print(f"fd={fd}: HIP FP32 {e_gpu:.2e}, oracle FP32 {e_ref32:.2e}")
+    assert rel_l2(o["gpu"][0], o["o64"][0]) < TOL_F32
+    assert e_gpu < 2.0 * max(e_ref32, TOL_F32)
+
+
+def _big_case():
+    """A grid large enough to have interior (mask-free) tiles."""
+    nz, nx, pad, nt = 96, 200, 12, 160
+    zz, xx = np.meshgrid(np.arange(nz), np.arange(nx), indexing="ij")
+    Vp = 2600.0 + 200.0 * np.sin(2 * np.pi * zz / 37) * np.cos(2 * np.pi * xx / 53)
+    Vs = Vp / 1.8
+    rho = 2100.0 + 90.0 * np.cos(np.pi * xx / 41)
+    m = dict(nz=nz, nx=nx, pad=pad, nt=nt, dx=10.0, dz=10.0, dt=1e-3, fc=20.0, Vs=Vs, Vp=Vp, rho=rho)
+    rx = np.arange(0, nx, 3)
+    rz = np.full_like(rx, 1)
+    wav = osig.ricker(np.arange(nt) * m["dt"], m["fc"], 1.5 / m["fc"]) * 1e6
+    return m, rx, rz, wav
+
+
+@pytest.mark.parametrize("imaging", ["exact", "zhou"])
+def test_gpu_interior_tiles_match_oracle_f64(imaging):
+    m, rx, rz, wav = _big_case()
+    fd, fs = 8, True
+    sx, sz = [57, 140], [5, 9]
+    rng = np.random.default_rng(3)
+    obs = rng.standard_normal((2, m["nt"], rx.size)) * 1e-9
+    p = _plan(m, fd, fs, torch.float64, nshots=2, nrec=rx.size, imaging=imaging, grad="all", hist=True)
+    p.set_model(m["Vs"], m["Vp"], m["rho"])
+    p.set_shots(sx, sz, ["moment", "force"], m["nt"], rx, rz, wav)
+    rec = p.forward()["rec"]
+    loss, radj = p.residual(rec, torch.as_tensor(obs), 2, None, 0, 1e15, 2)
+    p.adjoint(radj)
+    gg = p.finalize(m["Vs"], m["Vp"], m["rho"]).cpu().numpy()
+    g = oracle.Geom(m["nz"], m["nx"], m["pad"], fs, fd, m["nt"], m["dx"], m["dz"], m["dt"], m["fc"])
+    tape = ("e", "uv") if imaging == "zhou" else ("e", "dxu0", "s")
+    go = np.zeros_like(gg)
+    for k, st in enumerate(["moment", "force"]):
+        rec_o, t = oracle.forward(g, m["Vs"], m["Vp"], m["rho"], sx[k], sz[k], wav, rx, rz, st, np.float64, tape)
+        assert rel_l2(rec[k].cpu().numpy(), rec_o) < TOL_F64
+        _, cot = osig.residual_prep(rec_o[2], obs[k], None, 0, 1e15, 2)
+        ra = np.zeros_like(rec_o)
+        ra[2] = cot
+        a = oracle.adjoint(g, m["Vs"], m["Vp"], m["rho"], sx[k], sz[k], ra, rx, rz, st, wav, imaging, t)
+        chain = oracle.exact_chain if imaging == "exact" else oracle.zhou_chain
+        go += np.array(chain(g, a["g"], m["Vs"], m["Vp"], m["rho"]))
+    for i in range(3):
+        assert rel_l2(gg[i], go[i]) < TOL_F64, i
+
+
+def test_gpu_tile_heights_agree_bitwise():
+    """Every tile shape the autotuner can pick gives bitwise-identical
+    seismograms and gradients."""
+    m, rx, rz, wav = _big_case()
+    rng = np.random.default_rng(4)
+    obs = torch.as_tensor(rng.standard_normal((1, m["nt"], rx.size)) * 1e-9, dtype=torch.float32)
+    p = _plan(m, 8, True, torch.float32, nshots=1, nrec=rx.size, imaging="exact", grad="vs", hist=True)
+    p.set_model(m["Vs"], m["Vp"], m["rho"])
+    p.set_shots([77], [4], "moment", m["nt"], rx, rz, wav)
+    tiles, table = p.autotune(3)
+    assert all(t in (8, 16, 32) for t in tiles) and set(table) == {"fwd_vel", "fwd_strain", "adj_strain", "adj_vel"}
+    outs = []
+    for tz in ([8] * 4, [16] * 4, [32] * 4, [32, 8, 16, 8]):
+        p.set_tiles(tz)
+        rec = p.forward()["rec"]
+        _, radj = p.residual(rec, obs, 2, None, 0, 1e15, 1)
+        p.adjoint(radj)
+        outs.append((rec.clone(), p.finalize(m["Vs"], m["Vp"], m["rho"]).clone()))
+    for r, g in outs[1:]:
+        assert torch.equal(r, outs[0][0])
+        assert torch.equal(g, outs[0][1])
