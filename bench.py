@@ -108,6 +108,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--imaging", default="exact", choices=["exact", "zhou"])
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="after one forward, time each time-step kernel (20 launches) and a "
+                         "256 MiB device copy; for hardware-counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU design "
@@ -122,6 +125,18 @@ def main():
     f32 = torch.float32
     plan = Plan(NZ, NX, DX, DZ, DT, FC, PAD, fd_order=FD, free_surface=True, dtype=f32, n_shots_max=NSHOT,
                 nt_max=NT, nrec_max=NREC, imaging=args.imaging, grad="vs", store_history=True)
+    if args.kernels_only:
+        # no observed data: time each time-step kernel on the plan's buffers
+        plan.set_model(*(torch.as_tensor(a, dtype=f32, device=dev) for a in sv["init"]))
+        plan.set_shots(sv["src_x"], sv["src_z"], "moment", NT, sv["rec_x"], sv["rec_z"], sv["wav"])
+        tiles, _ = plan.autotune(10)
+        res = {n: plan.time_kernel(i, 20) for i, n in enumerate(("fwd_vel", "fwd_strain", "adj_strain", "adj_vel"))}
+        a = torch.zeros(64 << 20, dtype=f32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)                       # calibration: 256 MiB read + 256 MiB written
+        torch.cuda.synchronize()
+        print(json.dumps({"kernels_only": res, "tiles": tiles, "copy_bytes": a.numel() * 4}))
+        return
     # observed data on the true model
     plan.set_model(*sv["true"])
     plan.set_shots(sv["src_x"], sv["src_z"], "moment", NT, sv["rec_x"], sv["rec_z"], sv["wav"])

@@ -171,6 +171,15 @@ int stfwi_eprecond3(const void *ws, int n_shots, int nz_t, int nx_t, int free_su
 int stfwi_sosfilt(void *x, int nt, int ntr, const double *sos, int n_sos, int n_taper, int transpose, int dtype,
                   void *stream);
 
+/* Zhou imaging of given history cubes (imaging_condition,
+ * fwi/imaging.py:83-169): ex, ez, es, ua, va (and optionally vx, vz, both or
+ * neither) are nt x nz x nx padded-grid cubes, b_full the padded buoyancy
+ * (nz x nx); out receives g_lambda, g_mu, g_rho_lame on the padded grid
+ * (3 x nz x nx). */
+int stfwi_zhou_imaging(const void *ex, const void *ez, const void *es, const void *ua, const void *va,
+                       const void *b_full, const void *vx, const void *vz, int nt, int nz, int nx, double dx,
+                       double dz, double dt, int dtype, void *out, void *stream);
+
 /* Diagnostics (no reference counterpart): time `reps` back-to-back launches
  * of one time-step kernel over all shots of the plan (which: 0 forward
  * velocity, 1 forward strain, 2 adjoint strain, 3 adjoint velocity with

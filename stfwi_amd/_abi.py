@@ -40,6 +40,8 @@ SIGNATURES = {
     "stfwi_eprecond3": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _VP, _VP,
                                   C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _VP, _VP, _VP]),
     "stfwi_sosfilt": (C.c_int, [_VP, C.c_int, C.c_int, _VP, C.c_int, C.c_int, C.c_int, C.c_int, _VP]),
+    "stfwi_zhou_imaging": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, C.c_int, C.c_int, C.c_int,
+                                     C.c_double, C.c_double, C.c_double, C.c_int, _VP, _VP]),
     "stfwi_residual": (C.c_int, [_VP, _VP, _VP, C.c_int, _VP, C.c_int, C.c_int, C.c_double, C.c_int,
                                  _VP, _VP, _VP]),
 }

@@ -220,10 +220,14 @@ struct FwdArgs {
 // template parameter the autotuner picks) with 256 threads (64 x 4, each
 // thread owns TZ/4 rows).  Phase 1 stages what the stencils read (stress
 // products, velocities, or the C-PML-transposed derivative adjoints) for
-// the tile plus a 4-cell halo in LDS, computing each staged value once;
-// phase 2 evaluates the per-term-masked stencils from LDS.  Tiles whose
-// 8-cell neighbourhood is away from every grid edge, PML strip and the
-// free-surface row run a mask-free instantiation of the same body.
+// the tile plus a 4-cell halo in LDS, computing each staged value once: a
+// z-slab of TZ+8 rows over the tile's columns (whose middle rows also fill
+// the tile part of the x-slab) and the 4+4 halo columns of the x-slab.
+// Phase 2 evaluates the per-term-masked stencils from LDS.  A tile whose
+// columns lie 8 cells inside the x-PML strips is "x-interior" and one whose
+// rows lie 8 cells inside the z-PML strips and below the free-surface row is
+// "z-interior"; the masks and C-PML work along an interior axis are compiled
+// out (four instantiations of each body).
 
 // XCD-aware block order: workgroup n is dispatched to XCD n % 8, so give
 // each XCD a contiguous range of logical blocks; neighbouring tiles (which
@@ -241,17 +245,31 @@ __device__ __forceinline__ void logical_block(int &bx, int &bz)
     bx = lg - bz * gx;
 }
 
-// tile [r0, r0+TZ) x [c0, c0+64) is "interior" when every cell within 8 of
-// it is inside the grid, off the PML strips and off the free-surface row:
-// all per-term masks are then true and the C-PML recursions are no-ops
 template <typename T>
-__device__ __forceinline__ bool interior_tile(const Geo<T> &g, int r0, int c0, int tz)
+__device__ __forceinline__ bool interior_x(const Geo<T> &g, int c0)
 {
-    const int top = g.fs ? 8 : (g.pad + 8 > 8 ? g.pad + 8 : 8);
-    return r0 >= top && r0 + tz + 8 <= g.nz - g.pad && c0 >= g.pad + 8 && c0 + TILE_X + 8 <= g.nx - g.pad;
+    return c0 >= g.pad + 8 && c0 + TILE_X + 8 <= g.nx - g.pad;
 }
 
-#define CND(x) (INT || (x))
+template <typename T>
+__device__ __forceinline__ bool interior_z(const Geo<T> &g, int r0, int tz)
+{
+    const int top = g.fs ? 8 : g.pad + 8;
+    return r0 >= top && r0 + tz + 8 <= g.nz - g.pad;
+}
+
+#define CX(x) (IX || (x))
+#define CZ(x) (IZ || (x))
+#define STFWI_DISPATCH(ix, iz, body)                                          \
+    do {                                                                      \
+        if (ix) {                                                             \
+            if (iz) body(std::true_type{}, std::true_type{});                 \
+            else body(std::true_type{}, std::false_type{});                   \
+        } else {                                                              \
+            if (iz) body(std::false_type{}, std::true_type{});                \
+            else body(std::false_type{}, std::false_type{});                  \
+        }                                                                     \
+    } while (0)
 
 // velocity half-step (fwi/forward.py:253-367) + exx/ezz receivers of the
 // previous step (strains are read-only here)
@@ -275,14 +293,13 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
     }
     if (it >= sh.nt) return;
     constexpr int RPT = TZ / BLOCK_Y;
-    constexpr int NXH = TZ * (TILE_X + 8), NZH = (TZ + 8) * TILE_X;
     __shared__ T sP1[TZ][TILE_X + 8], sP2[TZ][TILE_X + 8], sP3x[TZ][TILE_X + 8];
     __shared__ T sP3z[TZ + 8][TILE_X], sP4[TZ + 8][TILE_X], sP5[TZ + 8][TILE_X];
     constexpr int NK = FDC<FD>::NK;
     const int ld = g.ld, nz = g.nz, nx = g.nx;
     const int tz = bx / a.tiles_x, tx = bx - tz * a.tiles_x;
     const int r0 = tz * TZ, c0 = tx * TILE_X;
-    const int tid = threadIdx.y * TILE_X + threadIdx.x;
+    const int lx = threadIdx.x, ty = threadIdx.y, tid = ty * TILE_X + lx;
     T *st = a.state + s * a.sstride + g.origin;
     T *__restrict__ U = st;
     T *__restrict__ V = st + g.fstride;
@@ -294,81 +311,93 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
     const T *__restrict__ B = a.B + g.origin;
     // stress products: (lambda+2mu) exx, lambda ezz, 2 mu exz along x;
     // 2 mu exz, (lambda+2mu) ezz, lambda exx along z
+    {
+        const int c = c0 + lx;
 #pragma unroll
-    for (int k = 0; k < (NXH + 255) / 256; ++k) {
-        const int i = tid + 256 * k;
-        if (NXH % 256 == 0 || i < NXH) {
-            const int zi = i / (TILE_X + 8), xi = i - zi * (TILE_X + 8);
-            const int r = r0 + zi, c = c0 - 4 + xi;
-            T p1 = 0, p2 = 0, p3 = 0;
-            if (r < nz && c >= 0 && c < nx) {
-                const i64 p = (i64)r * ld + c;
-                const T l2m = L2M[p], mu = M[p];
-                p1 = l2m * ex[p];
-                p2 = (l2m - T(2) * mu) * ez[p];
+        for (int k = 0; k < (TZ + 8) / BLOCK_Y; ++k) {
+            const int zi = ty + BLOCK_Y * k, r = r0 - 4 + zi;
+            T p1 = 0, p2 = 0, p3 = 0, p4 = 0, p5 = 0;
+            if (r >= 0 && r < nz && c < nx) {
+                const int p = r * ld + c;
+                const T l2m = L2M[p], mu = M[p], exv = ex[p], ezv = ez[p];
+                const T lam = l2m - T(2) * mu;
+                p1 = l2m * exv;
+                p2 = lam * ezv;
                 p3 = T(2) * mu * es[p];
+                p4 = l2m * ezv;
+                p5 = lam * exv;
             }
-            sP1[zi][xi] = p1;
-            sP2[zi][xi] = p2;
-            sP3x[zi][xi] = p3;
+            sP3z[zi][lx] = p3;
+            sP4[zi][lx] = p4;
+            sP5[zi][lx] = p5;
+            if (k >= 1 && k < TZ / BLOCK_Y + 1) {
+                sP1[zi - 4][lx + 4] = p1;
+                sP2[zi - 4][lx + 4] = p2;
+                sP3x[zi - 4][lx + 4] = p3;
+            }
         }
-    }
 #pragma unroll
-    for (int k = 0; k < NZH / 256; ++k) {
-        const int i = tid + 256 * k;
-        const int zi = i / TILE_X, xi = i - zi * TILE_X;
-        const int r = r0 - 4 + zi, c = c0 + xi;
-        T p3 = 0, p4 = 0, p5 = 0;
-        if (r >= 0 && r < nz && c < nx) {
-            const i64 p = (i64)r * ld + c;
-            const T l2m = L2M[p], mu = M[p];
-            p3 = T(2) * mu * es[p];
-            p4 = l2m * ez[p];
-            p5 = (l2m - T(2) * mu) * ex[p];
+        for (int k = 0; k < (8 * TZ + 255) / 256; ++k) {
+            const int e = tid + 256 * k;
+            if ((8 * TZ) % 256 == 0 || e < 8 * TZ) {
+                const int zi = e >> 3, h = e & 7, xi = h < 4 ? h : h + TILE_X;
+                const int r = r0 + zi, cc = c0 - 4 + xi;
+                T p1 = 0, p2 = 0, p3 = 0;
+                if (r < nz && cc >= 0 && cc < nx) {
+                    const int p = r * ld + cc;
+                    const T l2m = L2M[p], mu = M[p];
+                    p1 = l2m * ex[p];
+                    p2 = (l2m - T(2) * mu) * ez[p];
+                    p3 = T(2) * mu * es[p];
+                }
+                sP1[zi][xi] = p1;
+                sP2[zi][xi] = p2;
+                sP3x[zi][xi] = p3;
+            }
         }
-        sP3z[zi][xi] = p3;
-        sP4[zi][xi] = p4;
-        sP5[zi][xi] = p5;
     }
     __syncthreads();
-    const int lx = threadIdx.x, c = c0 + lx;
-    auto body = [&](auto tag) {
-        constexpr bool INT = decltype(tag)::value;
+    const int c = c0 + lx;
+    auto body = [&](auto tagx, auto tagz) {
+        constexpr bool IX = decltype(tagx)::value, IZ = decltype(tagz)::value;
 #pragma unroll
         for (int rr = 0; rr < RPT; ++rr) {
-            const int lz = threadIdx.y + BLOCK_Y * rr, r = r0 + lz;
-            if (!INT && (r > nz - 2 || c < 1 || c > nx - 2)) continue;
-            const i64 q = (i64)r * ld + c;
+            const int lz = ty + BLOCK_Y * rr, r = r0 + lz;
+            if ((!IZ && r > nz - 2) || (!IX && (c < 1 || c > nx - 2))) continue;
+            const int q = r * ld + c;
             const int xc = lx + 4, zc = lz + 4;
             T D1 = 0, D2 = 0, D3 = 0, D4 = 0, D5 = 0, D6 = 0;
 #pragma unroll
             for (int k = 1; k <= NK; ++k) {
                 const T ck = (T)fdc<FD>(k);
-                // D+x, term range [k-1, nx-1-k] (term 1: [0, nx-2], always true here)
-                if (CND(c >= k - 1 && c <= nx - 1 - k)) {
+                // D+x, term range [k-1, nx-1-k]
+                if (CX(c >= k - 1 && c <= nx - 1 - k)) {
                     D1 += ck * (sP1[lz][xc + k] - sP1[lz][xc - k + 1]);
                     D2 += ck * (sP2[lz][xc + k] - sP2[lz][xc - k + 1]);
                 }
                 // D-z of 2 mu exz, term 1 rows [1, nz-2], term k rows [k, nz-k]
-                if (CND(k == 1 ? (r >= 1 && r <= nz - 2) : (r >= k && r <= nz - k)))
+                if (CZ(k == 1 ? (r >= 1 && r <= nz - 2) : (r >= k && r <= nz - k)))
                     D3 += ck * (sP3z[zc + k - 1][lx] - sP3z[zc - k][lx]);
                 // D-x of 2 mu exz, term 1 cols [1, nx-2], term k cols [k, nx-k]
-                if (CND(k == 1 || (c >= k && c <= nx - k))) D4 += ck * (sP3x[lz][xc + k - 1] - sP3x[lz][xc - k]);
+                if (CX(k == 1 || (c >= k && c <= nx - k))) D4 += ck * (sP3x[lz][xc + k - 1] - sP3x[lz][xc - k]);
                 // D+z, term range [k-1, nz-1-k]
-                if (CND(r >= k - 1 && r <= nz - 1 - k)) {
+                if (CZ(r >= k - 1 && r <= nz - 1 - k)) {
                     D5 += ck * (sP4[zc + k][lx] - sP4[zc - k + 1][lx]);
                     D6 += ck * (sP5[zc + k][lx] - sP5[zc - k + 1][lx]);
                 }
             }
-            if (!INT) {
-                if (g.fs && r == 0) D3 = T(2 * fdc<FD>(1)) * sP3z[zc][lx];   // image of sigma_xz at the surface
-                const int ix = xstrip(g, r, c), iz = zstrip(g, r, c);
+            if (!IZ && g.fs && r == 0) D3 = T(2 * fdc<FD>(1)) * sP3z[zc][lx];   // image of sigma_xz at the surface
+            if (!IX) {
+                const int ix = xstrip(g, r, c);
                 if (ix >= 0) {
                     T *px = a.psix + s * a.psix_stride + ix;
                     D1 = pml_fwd(D1, px + 0 * a.psix_n, a.profx, c, nx);
                     D2 = pml_fwd(D2, px + 1 * a.psix_n, a.profx, c, nx);
                     D4 = pml_fwd(D4, px + 2 * a.psix_n, a.profx, c, nx);
                 }
+            }
+            if (!IZ) {
+                const int iz = zstrip(g, r, c);
                 if (iz >= 0) {
                     T *pz = a.psiz + s * a.psiz_stride + iz;
                     D3 = pml_fwd(D3, pz + 0 * a.psiz_n, a.profz, r, nz);
@@ -388,7 +417,7 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
             V[q] = v;
             if (a.hist2_mode == 1) {
                 const int rp = r - g.z0, cp = c - g.pad;
-                if (CND(rp >= 0 && rp < g.nzd && cp >= 0 && cp < g.nxd)) {
+                if (CZ(rp >= 0 && rp < g.nzd) && CX(cp >= 0 && cp < g.nxd)) {
                     T *h = a.hist2 + ((i64)s * a.nt_max + it) * 2 * g.np + (i64)rp * g.nxd + cp;
                     h[0] = g.dtdx * D1 + g.dtdx * D2 + g.dtdz * D3;
                     h[g.np] = g.dtdx * D4 + g.dtdz * D5 + g.dtdz * D6 + sv;
@@ -396,8 +425,7 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
             }
         }
     };
-    if (interior_tile(g, r0, c0, TZ)) body(std::true_type{});
-    else body(std::false_type{});
+    STFWI_DISPATCH(interior_x(g, c0), interior_z(g, r0, TZ), body);
 }
 
 // strain half-step (fwi/forward.py:369-444) + vx/vz receivers, history,
@@ -421,71 +449,85 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
         return;
     }
     constexpr int RPT = TZ / BLOCK_Y;
-    constexpr int NB = (TZ + 8) * (TILE_X + 8);
     __shared__ T sU[TZ + 8][TILE_X + 8], sV[TZ + 8][TILE_X + 8];
     constexpr int NK = FDC<FD>::NK;
     const int ld = g.ld, nz = g.nz, nx = g.nx;
     const int tz = bx / a.tiles_x, tx = bx - tz * a.tiles_x;
     const int r0 = tz * TZ, c0 = tx * TILE_X;
-    const int tid = threadIdx.y * TILE_X + threadIdx.x;
+    const int lx = threadIdx.x, ty = threadIdx.y, tid = ty * TILE_X + lx;
     T *st = a.state + s * a.sstride + g.origin;
     const T *__restrict__ U = st;
     const T *__restrict__ V = st + g.fstride;
     T *__restrict__ ex = st + 2 * g.fstride;
     T *__restrict__ ez = st + 3 * g.fstride;
     T *__restrict__ es = st + 4 * g.fstride;
+    {
+        const int c = c0 + lx;
 #pragma unroll
-    for (int k = 0; k < (NB + 255) / 256; ++k) {
-        const int i = tid + 256 * k;
-        if (NB % 256 == 0 || i < NB) {
-            const int zi = i / (TILE_X + 8), xi = i - zi * (TILE_X + 8);
-            const int r = r0 - 4 + zi, c = c0 - 4 + xi;
+        for (int k = 0; k < (TZ + 8) / BLOCK_Y; ++k) {
+            const int zi = ty + BLOCK_Y * k, r = r0 - 4 + zi;
             T u = 0, v = 0;
-            if (r >= 0 && r < nz && c >= 0 && c < nx) {
-                const i64 p = (i64)r * ld + c;
-                u = U[p];
-                v = V[p];
+            if (r >= 0 && r < nz && c < nx) {
+                u = U[r * ld + c];
+                v = V[r * ld + c];
             }
-            sU[zi][xi] = u;
-            sV[zi][xi] = v;
+            sU[zi][lx + 4] = u;
+            sV[zi][lx + 4] = v;
+        }
+#pragma unroll
+        for (int k = 0; k < (8 * (TZ + 8) + 255) / 256; ++k) {
+            const int e = tid + 256 * k;
+            if ((8 * (TZ + 8)) % 256 == 0 || e < 8 * (TZ + 8)) {
+                const int zi = e >> 3, h = e & 7, xi = h < 4 ? h : h + TILE_X;
+                const int r = r0 - 4 + zi, cc = c0 - 4 + xi;
+                T u = 0, v = 0;
+                if (r >= 0 && r < nz && cc >= 0 && cc < nx) {
+                    u = U[r * ld + cc];
+                    v = V[r * ld + cc];
+                }
+                sU[zi][xi] = u;
+                sV[zi][xi] = v;
+            }
         }
     }
     __syncthreads();
-    const int lx = threadIdx.x, c = c0 + lx;
-    auto body = [&](auto tag) {
-        constexpr bool INT = decltype(tag)::value;
+    const int c = c0 + lx;
+    auto body = [&](auto tagx, auto tagz) {
+        constexpr bool IX = decltype(tagx)::value, IZ = decltype(tagz)::value;
 #pragma unroll
         for (int rr = 0; rr < RPT; ++rr) {
-            const int lz = threadIdx.y + BLOCK_Y * rr, r = r0 + lz;
-            if (!INT && (r >= nz || c >= nx)) continue;
+            const int lz = ty + BLOCK_Y * rr, r = r0 + lz;
+            if ((!IZ && r >= nz) || (!IX && c >= nx)) continue;
             const int xc = lx + 4, zc = lz + 4;
-            const i64 q = (i64)r * ld + c;
-            const int ix = INT ? -1 : xstrip(g, r, c), iz = INT ? -1 : zstrip(g, r, c);
-            T *px = a.psix + s * a.psix_stride + ix;
-            T *pz = a.psiz + s * a.psiz_stride + iz;
+            const int q = r * ld + c;
             // exx: D-x U on columns [1, nx-1]
             T dxu = 0;
-            if (CND(c >= 1)) {
+            if (CX(c >= 1)) {
 #pragma unroll
                 for (int k = 1; k <= NK; ++k)
-                    if (CND(k == 1 || (c >= k && c <= nx - k)))
+                    if (CX(k == 1 || (c >= k && c <= nx - k)))
                         dxu += (T)fdc<FD>(k) * (sU[zc][xc + k - 1] - sU[zc][xc - k]);
             }
-            if (!INT && ix >= 0) dxu = pml_fwd(dxu, px + 3 * a.psix_n, a.profx, c, nx);
-            T exn = CND(c >= 1) ? ex[q] + g.dtdx * dxu : ex[q];
+            int ix = -1, iz = -1;
+            if (!IX) ix = xstrip(g, r, c);
+            if (!IZ) iz = zstrip(g, r, c);
+            T *px = a.psix + s * a.psix_stride + ix;
+            T *pz = a.psiz + s * a.psiz_stride + iz;
+            if (!IX && ix >= 0) dxu = pml_fwd(dxu, px + 3 * a.psix_n, a.profx, c, nx);
+            T exn = CX(c >= 1) ? ex[q] + g.dtdx * dxu : ex[q];
             // ezz: D-z V on columns [0, nx-2]; free-surface row from the C-PML'd D-x U
             T dzv = 0;
-            if (CND(c <= nx - 2)) {
-                if (!INT && g.fs && r == 0) {
+            if (CX(c <= nx - 2)) {
+                if (!IZ && g.fs && r == 0) {
                     dzv = a.Lr0[c] * dxu * g.dz / g.dx;
                 } else {
 #pragma unroll
                     for (int k = 1; k <= NK; ++k)
-                        if (CND(k == 1 ? r >= 1 : (r >= k && r <= nz - k)))
+                        if (CZ(k == 1 ? r >= 1 : (r >= k && r <= nz - k)))
                             dzv += (T)fdc<FD>(k) * (sV[zc + k - 1][xc] - sV[zc - k][xc]);
                 }
             }
-            if (!INT && iz >= 0) dzv = pml_fwd(dzv, pz + 3 * a.psiz_n, a.profz, r, nz);
+            if (!IZ && iz >= 0) dzv = pml_fwd(dzv, pz + 3 * a.psiz_n, a.profz, r, nz);
             T ezn = ez[q] + g.dtdz * dzv;
             if (sh.type == 1 && r == sh.sz && c == sh.sx) {
                 const T lm = (a.L2M[g.origin + q] - T(2) * a.M[g.origin + q]) + a.M[g.origin + q];
@@ -497,22 +539,22 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
             ez[q] = ezn;
             // exz on [0, nz-2] x [0, nx-2]
             T esn = es[q];
-            if (CND(r <= nz - 2 && c <= nx - 2)) {
+            if (CZ(r <= nz - 2) && CX(c <= nx - 2)) {
                 T dzu = 0, dxv = 0;
 #pragma unroll
                 for (int k = 1; k <= NK; ++k) {
                     const T ck = (T)fdc<FD>(k);
-                    if (CND(c >= 1 && r >= k - 1 && r <= nz - 1 - k)) dzu += ck * (sU[zc + k][xc] - sU[zc - k + 1][xc]);
-                    if (CND(c >= k - 1 && c <= nx - 1 - k)) dxv += ck * (sV[zc][xc + k] - sV[zc][xc - k + 1]);
+                    if (CX(c >= 1) && CZ(r >= k - 1 && r <= nz - 1 - k)) dzu += ck * (sU[zc + k][xc] - sU[zc - k + 1][xc]);
+                    if (CX(c >= k - 1 && c <= nx - 1 - k)) dxv += ck * (sV[zc][xc + k] - sV[zc][xc - k + 1]);
                 }
-                if (!INT && iz >= 0) dzu = pml_fwd(dzu, pz + 4 * a.psiz_n, a.profz, r, nz);
-                if (!INT && ix >= 0) dxv = pml_fwd(dxv, px + 4 * a.psix_n, a.profx, c, nx);
+                if (!IZ && iz >= 0) dzu = pml_fwd(dzu, pz + 4 * a.psiz_n, a.profz, r, nz);
+                if (!IX && ix >= 0) dxv = pml_fwd(dxv, px + 4 * a.psix_n, a.profx, c, nx);
                 esn = esn + T(0.5) * (g.dtdz * dzu + g.dtdx * dxv);
                 es[q] = esn;
             }
-            if (!INT && a.dxu0 && r == 0) a.dxu0[((i64)s * a.nt_max + it) * nx + c] = dxu;
+            if (!IZ && a.dxu0 && r == 0) a.dxu0[((i64)s * a.nt_max + it) * nx + c] = dxu;
             const int rp = r - g.z0, cp = c - g.pad;
-            if (a.hist && CND(rp >= 0 && rp < g.nzd && cp >= 0 && cp < g.nxd)) {
+            if (a.hist && CZ(rp >= 0 && rp < g.nzd) && CX(cp >= 0 && cp < g.nxd)) {
                 const i64 o = ((i64)s * a.nt_max + it) * 3 * g.np + (i64)rp * g.nxd + cp;
                 a.hist[o] = exn;
                 a.hist[o + g.np] = ezn;
@@ -539,8 +581,7 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
             }
         }
     };
-    if (interior_tile(g, r0, c0, TZ)) body(std::true_type{});
-    else body(std::false_type{});
+    STFWI_DISPATCH(interior_x(g, c0), interior_z(g, r0, TZ), body);
 }
 
 
@@ -607,13 +648,13 @@ struct K4Src {
     const T *Lr0;
     __device__ __forceinline__ T GzU(int r, int c) const
     {
-        return (r <= g.nz - 2 && c <= g.nx - 2) ? g.hdtdz * esa[(i64)r * g.ld + c] : T(0);
+        return (r <= g.nz - 2 && c <= g.nx - 2) ? g.hdtdz * esa[r * g.ld + c] : T(0);
     }
     __device__ __forceinline__ T GxV(int r, int c) const
     {
-        return (r <= g.nz - 2 && c <= g.nx - 2) ? g.hdtdx * esa[(i64)r * g.ld + c] : T(0);
+        return (r <= g.nz - 2 && c <= g.nx - 2) ? g.hdtdx * esa[r * g.ld + c] : T(0);
     }
-    __device__ __forceinline__ T GzV(int r, int c) const { return g.dtdz * eza[(i64)r * g.ld + c]; }
+    __device__ __forceinline__ T GzV(int r, int c) const { return g.dtdz * eza[r * g.ld + c]; }
     __device__ __forceinline__ T DzVa(int r, int c) const
     {
         const T G = GzV(r, c);
@@ -624,7 +665,7 @@ struct K4Src {
     {
         T G = 0;
         if (g.fs && r == 0 && c <= g.nx - 2) G = Lr0[c] * g.dzdx * DzVa(0, c);
-        if (c >= 1) G = G + g.dtdx * exa[(i64)r * g.ld + c];
+        if (c >= 1) G = G + g.dtdx * exa[r * g.ld + c];
         return G;
     }
     __device__ __forceinline__ T DzUa(int r, int c) const
@@ -658,113 +699,122 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
     const ShotDev sh = a.shots[s];
     if (it >= sh.nt) return;
     constexpr int RPT = TZ / BLOCK_Y;
-    constexpr int NXH = TZ * (TILE_X + 8), NZH = (TZ + 8) * TILE_X;
     __shared__ T szU[TZ + 8][TILE_X], szV[TZ + 8][TILE_X];
     __shared__ T sxV[TZ][TILE_X + 8], sxU[TZ][TILE_X + 8];
     constexpr int NK = FDC<FD>::NK;
-    const int nz = g.nz, nx = g.nx;
+    const int nz = g.nz, nx = g.nx, ld = g.ld;
     const int tz = bx / a.tiles_x, tx = bx - tz * a.tiles_x;
     const int r0 = tz * TZ, c0 = tx * TILE_X;
-    const int tid = threadIdx.y * TILE_X + threadIdx.x;
+    const int lx = threadIdx.x, ty = threadIdx.y, tid = ty * TILE_X + lx;
     T *st = a.state + s * a.sstride + g.origin;
     T *Ua = st, *Va = st + g.fstride;
     const K4Src<T> S{g, st + 4 * g.fstride, st + 2 * g.fstride, st + 3 * g.fstride,
                      a.psix_o + s * a.psix_stride, a.psiz_o + s * a.psiz_stride,
                      a.psix_len, a.psiz_len, a.profx, a.profz, a.Lr0};
-    const bool inter = interior_tile(g, r0, c0, TZ);
-    const int lx = threadIdx.x, c = c0 + lx;
+    const bool inx = interior_x(g, c0), inz = interior_z(g, r0, TZ);
+    const int c = c0 + lx;
     // owned values, loaded ahead of the barrier
     T ua0[RPT], va0[RPT];
 #pragma unroll
     for (int rr = 0; rr < RPT; ++rr) {
-        const int r = r0 + threadIdx.y + BLOCK_Y * rr;
+        const int r = r0 + ty + BLOCK_Y * rr;
         const bool ok = r < nz && c < nx;
-        ua0[rr] = ok ? Ua[(i64)r * g.ld + c] : T(0);
-        va0[rr] = ok ? Va[(i64)r * g.ld + c] : T(0);
+        ua0[rr] = ok ? Ua[r * ld + c] : T(0);
+        va0[rr] = ok ? Va[r * ld + c] : T(0);
     }
     // derivative adjoints after the transposed C-PML, tile + halo
-    if (inter) {
+    if (inx && inz) {
         // no PML, no masks: the adjoints are the scaled strain adjoints
 #pragma unroll
-        for (int k = 0; k < NZH / 256; ++k) {
-            const int i = tid + 256 * k;
-            const int zi = i / TILE_X, xi = i - zi * TILE_X;
-            const i64 p = (i64)(r0 - 4 + zi) * g.ld + c0 + xi;
-            szU[zi][xi] = g.hdtdz * S.esa[p];
-            szV[zi][xi] = g.dtdz * S.eza[p];
+        for (int k = 0; k < (TZ + 8) / BLOCK_Y; ++k) {
+            const int zi = ty + BLOCK_Y * k;
+            const int p = (r0 - 4 + zi) * ld + c;
+            const T e_s = S.esa[p];
+            szU[zi][lx] = g.hdtdz * e_s;
+            szV[zi][lx] = g.dtdz * S.eza[p];
+            if (k >= 1 && k < TZ / BLOCK_Y + 1) {
+                sxV[zi - 4][lx + 4] = g.hdtdx * e_s;
+                sxU[zi - 4][lx + 4] = g.dtdx * S.exa[p];
+            }
         }
 #pragma unroll
-        for (int k = 0; k < (NXH + 255) / 256; ++k) {
-            const int i = tid + 256 * k;
-            if (NXH % 256 == 0 || i < NXH) {
-                const int zi = i / (TILE_X + 8), xi = i - zi * (TILE_X + 8);
-                const i64 p = (i64)(r0 + zi) * g.ld + c0 - 4 + xi;
+        for (int k = 0; k < (8 * TZ + 255) / 256; ++k) {
+            const int e = tid + 256 * k;
+            if ((8 * TZ) % 256 == 0 || e < 8 * TZ) {
+                const int zi = e >> 3, h = e & 7, xi = h < 4 ? h : h + TILE_X;
+                const int p = (r0 + zi) * ld + c0 - 4 + xi;
                 sxV[zi][xi] = g.hdtdx * S.esa[p];
                 sxU[zi][xi] = g.dtdx * S.exa[p];
             }
         }
     } else {
 #pragma unroll
-        for (int k = 0; k < NZH / 256; ++k) {
-            const int i = tid + 256 * k;
-            const int zi = i / TILE_X, xi = i - zi * TILE_X;
-            const int r = r0 - 4 + zi, c1 = c0 + xi;
-            T u = 0, v = 0;
-            if (r >= 0 && r < nz && c1 < nx) {
-                u = S.DzUa(r, c1);
-                v = S.DzVa(r, c1);
+        for (int k = 0; k < (TZ + 8) / BLOCK_Y; ++k) {
+            const int zi = ty + BLOCK_Y * k, r = r0 - 4 + zi;
+            T u = 0, v = 0, xv = 0, xu = 0;
+            if (r >= 0 && r < nz && c < nx) {
+                u = S.DzUa(r, c);
+                v = S.DzVa(r, c);
+                if (k >= 1 && k < TZ / BLOCK_Y + 1) {
+                    xv = S.DxVa(r, c);
+                    xu = S.DxUa(r, c);
+                }
             }
-            szU[zi][xi] = u;
-            szV[zi][xi] = v;
+            szU[zi][lx] = u;
+            szV[zi][lx] = v;
+            if (k >= 1 && k < TZ / BLOCK_Y + 1) {
+                sxV[zi - 4][lx + 4] = xv;
+                sxU[zi - 4][lx + 4] = xu;
+            }
         }
 #pragma unroll
-        for (int k = 0; k < (NXH + 255) / 256; ++k) {
-            const int i = tid + 256 * k;
-            if (NXH % 256 == 0 || i < NXH) {
-                const int zi = i / (TILE_X + 8), xi = i - zi * (TILE_X + 8);
-                const int r = r0 + zi, c1 = c0 - 4 + xi;
-                T u = 0, v = 0;
-                if (r < nz && c1 >= 0 && c1 < nx) {
-                    v = S.DxVa(r, c1);
-                    u = S.DxUa(r, c1);
+        for (int k = 0; k < (8 * TZ + 255) / 256; ++k) {
+            const int e = tid + 256 * k;
+            if ((8 * TZ) % 256 == 0 || e < 8 * TZ) {
+                const int zi = e >> 3, h = e & 7, xi = h < 4 ? h : h + TILE_X;
+                const int r = r0 + zi, cc = c0 - 4 + xi;
+                T xv = 0, xu = 0;
+                if (r < nz && cc >= 0 && cc < nx) {
+                    xv = S.DxVa(r, cc);
+                    xu = S.DxUa(r, cc);
                 }
-                sxV[zi][xi] = v;
-                sxU[zi][xi] = u;
+                sxV[zi][xi] = xv;
+                sxU[zi][xi] = xu;
             }
         }
     }
     __syncthreads();
-    auto body = [&](auto tag) {
-        constexpr bool INT = decltype(tag)::value;
+    auto body = [&](auto tagx, auto tagz) {
+        constexpr bool IX = decltype(tagx)::value, IZ = decltype(tagz)::value;
 #pragma unroll
         for (int rr = 0; rr < RPT; ++rr) {
-            const int lz = threadIdx.y + BLOCK_Y * rr, r = r0 + lz;
-            if (!INT && (r >= nz || c >= nx)) continue;
+            const int lz = ty + BLOCK_Y * rr, r = r0 + lz;
+            if ((!IZ && r >= nz) || (!IX && c >= nx)) continue;
             const int xc = lx + 4, zc = lz + 4;
-            const i64 q = (i64)r * g.ld + c;
+            const int q = r * ld + c;
             T ua = ua0[rr], va = va0[rr];
 #pragma unroll
             for (int k = 1; k <= NK; ++k) {
                 const T ck = (T)fdc<FD>(k);
                 // (D+z)^T of Dz_U_a, columns [1, nx-1]; term range [k-1, nz-1-k]
-                if (CND(c >= 1)) {
-                    if (CND(r - k >= k - 1 && r - k <= nz - 1 - k)) ua += ck * szU[zc - k][lx];
-                    if (CND(r + k - 1 >= k - 1 && r + k - 1 <= nz - 1 - k)) ua += -ck * szU[zc + k - 1][lx];
+                if (CX(c >= 1)) {
+                    if (CZ(r - k >= k - 1 && r - k <= nz - 1 - k)) ua += ck * szU[zc - k][lx];
+                    if (CZ(r + k - 1 >= k - 1 && r + k - 1 <= nz - 1 - k)) ua += -ck * szU[zc + k - 1][lx];
                 }
                 // (D+x)^T of Dx_V_a, rows [0, nz-2]
-                if (CND(r <= nz - 2)) {
-                    if (CND(c - k >= k - 1 && c - k <= nx - 1 - k)) va += ck * sxV[lz][xc - k];
-                    if (CND(c + k - 1 >= k - 1 && c + k - 1 <= nx - 1 - k)) va += -ck * sxV[lz][xc + k - 1];
+                if (CZ(r <= nz - 2)) {
+                    if (CX(c - k >= k - 1 && c - k <= nx - 1 - k)) va += ck * sxV[lz][xc - k];
+                    if (CX(c + k - 1 >= k - 1 && c + k - 1 <= nx - 1 - k)) va += -ck * sxV[lz][xc + k - 1];
                 }
             }
 #pragma unroll
             for (int k = 1; k <= NK; ++k) {
                 const T ck = (T)fdc<FD>(k);
                 // (D-z)^T of Dz_V_a, columns [0, nx-2]; term 1 rows [1, nz-1], k: [k, nz-k]
-                if (CND(c <= nx - 2)) {
+                if (CX(c <= nx - 2)) {
                     const int i1 = r - k + 1, i2 = r + k;
-                    if (CND(k == 1 ? (i1 >= 1 && i1 <= nz - 1) : (i1 >= k && i1 <= nz - k))) va += ck * szV[zc - k + 1][lx];
-                    if (CND(k == 1 ? (i2 >= 1 && i2 <= nz - 1) : (i2 >= k && i2 <= nz - k))) va += -ck * szV[zc + k][lx];
+                    if (CZ(k == 1 ? (i1 >= 1 && i1 <= nz - 1) : (i1 >= k && i1 <= nz - k))) va += ck * szV[zc - k + 1][lx];
+                    if (CZ(k == 1 ? (i2 >= 1 && i2 <= nz - 1) : (i2 >= k && i2 <= nz - k))) va += -ck * szV[zc + k][lx];
                 }
             }
 #pragma unroll
@@ -772,19 +822,22 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
                 const T ck = (T)fdc<FD>(k);
                 // (D-x)^T of Dx_U_a, all rows; term 1 cols [1, nx-1], k: [k, nx-k]
                 const int i1 = c - k + 1, i2 = c + k;
-                if (CND(k == 1 ? (i1 >= 1 && i1 <= nx - 1) : (i1 >= k && i1 <= nx - k))) ua += ck * sxU[lz][xc - k + 1];
-                if (CND(k == 1 ? (i2 >= 1 && i2 <= nx - 1) : (i2 >= k && i2 <= nx - k))) ua += -ck * sxU[lz][xc + k];
+                if (CX(k == 1 ? (i1 >= 1 && i1 <= nx - 1) : (i1 >= k && i1 <= nx - k))) ua += ck * sxU[lz][xc - k + 1];
+                if (CX(k == 1 ? (i2 >= 1 && i2 <= nx - 1) : (i2 >= k && i2 <= nx - k))) ua += -ck * sxU[lz][xc + k];
             }
             Ua[q] = ua;
             Va[q] = va;
-            if (!INT) {
-                // C-PML memories of this cell
-                const int ix = xstrip(g, r, c), iz = zstrip(g, r, c);
+            // C-PML memories of this cell
+            if (!IZ) {
+                const int iz = zstrip(g, r, c);
                 if (iz >= 0) {
                     T *pz = a.psiz_n + s * a.psiz_stride + iz;
                     pz[0] = pmlT_psi(S.GzU(r, c), S.pzo[iz], a.profz, r, nz);
                     pz[a.psiz_len] = pmlT_psi(S.GzV(r, c), S.pzo[a.psiz_len + iz], a.profz, r, nz);
                 }
+            }
+            if (!IX) {
+                const int ix = xstrip(g, r, c);
                 if (ix >= 0) {
                     T *px = a.psix_n + s * a.psix_stride + ix;
                     px[0] = pmlT_psi(S.GxV(r, c), S.pxo[ix], a.profx, c, nx);
@@ -803,8 +856,7 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
             }
         }
     };
-    if (inter) body(std::true_type{});
-    else body(std::false_type{});
+    STFWI_DISPATCH(inx, inz, body);
 }
 
 
@@ -819,22 +871,22 @@ struct K5Src {
     __device__ __forceinline__ bool inR(int r, int c) const { return r <= g.nz - 2 && c >= 1 && c <= g.nx - 2; }
     __device__ __forceinline__ T GxBU(int r, int c) const
     {
-        const i64 p = (i64)r * g.ld + c;
+        const int p = r * g.ld + c;
         return inR(r, c) ? g.dtdx * B[p] * Ua[p] : T(0);
     }
     __device__ __forceinline__ T GzMes(int r, int c) const
     {
-        const i64 p = (i64)r * g.ld + c;
+        const int p = r * g.ld + c;
         return inR(r, c) ? g.dtdz * B[p] * Ua[p] : T(0);
     }
     __device__ __forceinline__ T GxMes(int r, int c) const
     {
-        const i64 p = (i64)r * g.ld + c;
+        const int p = r * g.ld + c;
         return inR(r, c) ? g.dtdx * B[p] * Va[p] : T(0);
     }
     __device__ __forceinline__ T GzBV(int r, int c) const
     {
-        const i64 p = (i64)r * g.ld + c;
+        const int p = r * g.ld + c;
         return inR(r, c) ? g.dtdz * B[p] * Va[p] : T(0);
     }
     __device__ __forceinline__ T DxBUa(int r, int c) const
@@ -874,119 +926,127 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
     const ShotDev sh = a.shots[s];
     if (it >= sh.nt) return;
     constexpr int RPT = TZ / BLOCK_Y;
-    constexpr int NXH = TZ * (TILE_X + 8), NZH = (TZ + 8) * TILE_X;
     __shared__ T sxBU[TZ][TILE_X + 8], sxMes[TZ][TILE_X + 8];
     __shared__ T szBV[TZ + 8][TILE_X], szMes[TZ + 8][TILE_X];
     constexpr int NK = FDC<FD>::NK;
-    const int nz = g.nz, nx = g.nx;
+    const int nz = g.nz, nx = g.nx, ld = g.ld;
     const int tz = bx / a.tiles_x, tx = bx - tz * a.tiles_x;
     const int r0 = tz * TZ, c0 = tx * TILE_X;
-    const int tid = threadIdx.y * TILE_X + threadIdx.x;
+    const int lx = threadIdx.x, ty = threadIdx.y, tid = ty * TILE_X + lx;
     T *st = a.state + s * a.sstride + g.origin;
     const T *Ua = st, *Va = st + g.fstride;
     T *exa = st + 2 * g.fstride, *eza = st + 3 * g.fstride, *esa = st + 4 * g.fstride;
     const T *Bp = a.B + g.origin;
     const K5Src<T> S{g, Ua, Va, Bp, a.psix_o + s * a.psix_stride, a.psiz_o + s * a.psiz_stride,
                      a.psix_len, a.psiz_len, a.profx, a.profz};
-    const bool inter = interior_tile(g, r0, c0, TZ);
-    const int lx = threadIdx.x, c = c0 + lx;
-    if (inter) {
+    const bool inx = interior_x(g, c0), inz = interior_z(g, r0, TZ);
+    const int c = c0 + lx;
+    if (inx && inz) {
         // no PML, no masks: Dx_(BU)_a = dt/dx B U_a, Dz_M_es_a = dt/dz B U_a, ...
 #pragma unroll
-        for (int k = 0; k < (NXH + 255) / 256; ++k) {
-            const int i = tid + 256 * k;
-            if (NXH % 256 == 0 || i < NXH) {
-                const int zi = i / (TILE_X + 8), xi = i - zi * (TILE_X + 8);
-                const i64 p = (i64)(r0 + zi) * g.ld + c0 - 4 + xi;
+        for (int k = 0; k < (TZ + 8) / BLOCK_Y; ++k) {
+            const int zi = ty + BLOCK_Y * k;
+            const int p = (r0 - 4 + zi) * ld + c;
+            const T b = Bp[p], u = Ua[p], v = Va[p];
+            szBV[zi][lx] = g.dtdz * b * v;
+            szMes[zi][lx] = g.dtdz * b * u;
+            if (k >= 1 && k < TZ / BLOCK_Y + 1) {
+                sxBU[zi - 4][lx + 4] = g.dtdx * b * u;
+                sxMes[zi - 4][lx + 4] = g.dtdx * b * v;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < (8 * TZ + 255) / 256; ++k) {
+            const int e = tid + 256 * k;
+            if ((8 * TZ) % 256 == 0 || e < 8 * TZ) {
+                const int zi = e >> 3, h = e & 7, xi = h < 4 ? h : h + TILE_X;
+                const int p = (r0 + zi) * ld + c0 - 4 + xi;
                 const T b = Bp[p];
                 sxBU[zi][xi] = g.dtdx * b * Ua[p];
                 sxMes[zi][xi] = g.dtdx * b * Va[p];
             }
         }
-#pragma unroll
-        for (int k = 0; k < NZH / 256; ++k) {
-            const int i = tid + 256 * k;
-            const int zi = i / TILE_X, xi = i - zi * TILE_X;
-            const i64 p = (i64)(r0 - 4 + zi) * g.ld + c0 + xi;
-            const T b = Bp[p];
-            szBV[zi][xi] = g.dtdz * b * Va[p];
-            szMes[zi][xi] = g.dtdz * b * Ua[p];
-        }
     } else {
 #pragma unroll
-        for (int k = 0; k < (NXH + 255) / 256; ++k) {
-            const int i = tid + 256 * k;
-            if (NXH % 256 == 0 || i < NXH) {
-                const int zi = i / (TILE_X + 8), xi = i - zi * (TILE_X + 8);
-                const int r = r0 + zi, c1 = c0 - 4 + xi;
-                T u = 0, v = 0;
-                if (r < nz && c1 >= 0 && c1 < nx) {
-                    u = S.DxBUa(r, c1);
-                    v = S.DxMesa(r, c1);
+        for (int k = 0; k < (TZ + 8) / BLOCK_Y; ++k) {
+            const int zi = ty + BLOCK_Y * k, r = r0 - 4 + zi;
+            T bv = 0, mz = 0, bu = 0, mx = 0;
+            if (r >= 0 && r < nz && c < nx) {
+                bv = S.DzBVa(r, c);
+                mz = S.DzMesa(r, c);
+                if (k >= 1 && k < TZ / BLOCK_Y + 1) {
+                    bu = S.DxBUa(r, c);
+                    mx = S.DxMesa(r, c);
                 }
-                sxBU[zi][xi] = u;
-                sxMes[zi][xi] = v;
+            }
+            szBV[zi][lx] = bv;
+            szMes[zi][lx] = mz;
+            if (k >= 1 && k < TZ / BLOCK_Y + 1) {
+                sxBU[zi - 4][lx + 4] = bu;
+                sxMes[zi - 4][lx + 4] = mx;
             }
         }
 #pragma unroll
-        for (int k = 0; k < NZH / 256; ++k) {
-            const int i = tid + 256 * k;
-            const int zi = i / TILE_X, xi = i - zi * TILE_X;
-            const int r = r0 - 4 + zi, c1 = c0 + xi;
-            T u = 0, v = 0;
-            if (r >= 0 && r < nz && c1 < nx) {
-                v = S.DzBVa(r, c1);
-                u = S.DzMesa(r, c1);
+        for (int k = 0; k < (8 * TZ + 255) / 256; ++k) {
+            const int e = tid + 256 * k;
+            if ((8 * TZ) % 256 == 0 || e < 8 * TZ) {
+                const int zi = e >> 3, h = e & 7, xi = h < 4 ? h : h + TILE_X;
+                const int r = r0 + zi, cc = c0 - 4 + xi;
+                T bu = 0, mx = 0;
+                if (r < nz && cc >= 0 && cc < nx) {
+                    bu = S.DxBUa(r, cc);
+                    mx = S.DxMesa(r, cc);
+                }
+                sxBU[zi][xi] = bu;
+                sxMes[zi][xi] = mx;
             }
-            szBV[zi][xi] = v;
-            szMes[zi][xi] = u;
         }
     }
     __syncthreads();
-    auto body = [&](auto tag) {
-        constexpr bool INT = decltype(tag)::value;
+    auto body = [&](auto tagx, auto tagz) {
+        constexpr bool IX = decltype(tagx)::value, IZ = decltype(tagz)::value;
 #pragma unroll
         for (int rr = 0; rr < RPT; ++rr) {
-            const int lz = threadIdx.y + BLOCK_Y * rr, r = r0 + lz;
-            if (!INT && (r >= nz || c >= nx)) continue;
+            const int lz = ty + BLOCK_Y * rr, r = r0 + lz;
+            if ((!IZ && r >= nz) || (!IX && c >= nx)) continue;
             const int xc = lx + 4, zc = lz + 4;
-            const i64 q = (i64)r * g.ld + c;
+            const int q = r * ld + c;
             T Ax = 0, Az = 0, Mes = 0;
 #pragma unroll
             for (int k = 1; k <= NK; ++k) {
                 const T ck = (T)fdc<FD>(k);
                 // (D+x)^T of Dx_(B U)_a (= L2M_ex_a = L_ez_a), rows [0, nz-2]
-                if (CND(r <= nz - 2)) {
-                    if (CND(c - k >= k - 1 && c - k <= nx - 1 - k)) Ax += ck * sxBU[lz][xc - k];
-                    if (CND(c + k - 1 >= k - 1 && c + k - 1 <= nx - 1 - k)) Ax += -ck * sxBU[lz][xc + k - 1];
+                if (CZ(r <= nz - 2)) {
+                    if (CX(c - k >= k - 1 && c - k <= nx - 1 - k)) Ax += ck * sxBU[lz][xc - k];
+                    if (CX(c + k - 1 >= k - 1 && c + k - 1 <= nx - 1 - k)) Ax += -ck * sxBU[lz][xc + k - 1];
                 }
                 // (D+z)^T of Dz_(B V)_a (= L2M_ez_a = L_ex_a), columns [1, nx-2]
-                if (CND(c >= 1 && c <= nx - 2)) {
-                    if (CND(r - k >= k - 1 && r - k <= nz - 1 - k)) Az += ck * szBV[zc - k][lx];
-                    if (CND(r + k - 1 >= k - 1 && r + k - 1 <= nz - 1 - k)) Az += -ck * szBV[zc + k - 1][lx];
+                if (CX(c >= 1 && c <= nx - 2)) {
+                    if (CZ(r - k >= k - 1 && r - k <= nz - 1 - k)) Az += ck * szBV[zc - k][lx];
+                    if (CZ(r + k - 1 >= k - 1 && r + k - 1 <= nz - 1 - k)) Az += -ck * szBV[zc + k - 1][lx];
                 }
             }
 #pragma unroll
             for (int k = 1; k <= NK; ++k) {
                 const T ck = (T)fdc<FD>(k);
                 // (D-x)^T of Dx_M_es_a, rows [0, nz-2]; term 1 cols [1, nx-2], k: [k, nx-k]
-                if (CND(r <= nz - 2)) {
+                if (CZ(r <= nz - 2)) {
                     const int i1 = c - k + 1, i2 = c + k;
-                    if (CND(k == 1 ? (i1 >= 1 && i1 <= nx - 2) : (i1 >= k && i1 <= nx - k))) Mes += ck * sxMes[lz][xc - k + 1];
-                    if (CND(k == 1 ? (i2 >= 1 && i2 <= nx - 2) : (i2 >= k && i2 <= nx - k))) Mes += -ck * sxMes[lz][xc + k];
+                    if (CX(k == 1 ? (i1 >= 1 && i1 <= nx - 2) : (i1 >= k && i1 <= nx - k))) Mes += ck * sxMes[lz][xc - k + 1];
+                    if (CX(k == 1 ? (i2 >= 1 && i2 <= nx - 2) : (i2 >= k && i2 <= nx - k))) Mes += -ck * sxMes[lz][xc + k];
                 }
             }
 #pragma unroll
             for (int k = 1; k <= NK; ++k) {
                 const T ck = (T)fdc<FD>(k);
                 // (D-z)^T of Dz_M_es_a, columns [1, nx-2]; term 1 rows [1, nz-2], k: [k, nz-k]
-                if (CND(c >= 1 && c <= nx - 2)) {
+                if (CX(c >= 1 && c <= nx - 2)) {
                     const int i1 = r - k + 1, i2 = r + k;
-                    if (CND(k == 1 ? (i1 >= 1 && i1 <= nz - 2) : (i1 >= k && i1 <= nz - k))) Mes += ck * szMes[zc - k + 1][lx];
-                    if (CND(k == 1 ? (i2 >= 1 && i2 <= nz - 2) : (i2 >= k && i2 <= nz - k))) Mes += -ck * szMes[zc + k][lx];
+                    if (CZ(k == 1 ? (i1 >= 1 && i1 <= nz - 2) : (i1 >= k && i1 <= nz - k))) Mes += ck * szMes[zc - k + 1][lx];
+                    if (CZ(k == 1 ? (i2 >= 1 && i2 <= nz - 2) : (i2 >= k && i2 <= nz - k))) Mes += -ck * szMes[zc + k][lx];
                 }
             }
-            if (!INT && g.fs && r == 0 && c >= 1 && c <= nx - 2) Mes += T(2 * fdc<FD>(1)) * szMes[zc][lx];
+            if (!IZ && g.fs && r == 0 && c >= 1 && c <= nx - 2) Mes += T(2 * fdc<FD>(1)) * szMes[zc][lx];
 
             const T l2m = a.L2M[g.origin + q], mu = a.M[g.origin + q];
             const T lam = l2m - T(2) * mu;
@@ -995,13 +1055,16 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
             eza[q] = eza0 + lam * Ax + l2m * Az;
             esa[q] = esa[q] + T(2) * mu * Mes;
 
-            if (!INT) {
-                const int ix = xstrip(g, r, c), iz = zstrip(g, r, c);
+            if (!IX) {
+                const int ix = xstrip(g, r, c);
                 if (ix >= 0) {
                     T *px = a.psix_n + s * a.psix_stride + ix;
                     px[2 * a.psix_len] = pmlT_psi(S.GxMes(r, c), S.pxo[2 * a.psix_len + ix], a.profx, c, nx);
                     px[3 * a.psix_len] = pmlT_psi(S.GxBU(r, c), S.pxo[3 * a.psix_len + ix], a.profx, c, nx);
                 }
+            }
+            if (!IZ) {
+                const int iz = zstrip(g, r, c);
                 if (iz >= 0) {
                     T *pz = a.psiz_n + s * a.psiz_stride + iz;
                     pz[2 * a.psiz_len] = pmlT_psi(S.GzBV(r, c), S.pzo[2 * a.psiz_len + iz], a.profz, r, nz);
@@ -1021,7 +1084,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
 
             // imaging on the physical cells
             const int rp = r - g.z0, cp = c - g.pad;
-            if (a.imaging == 0 || !CND(rp >= 0 && rp < g.nzd && cp >= 0 && cp < g.nxd)) continue;
+            if (a.imaging == 0 || !(CZ(rp >= 0 && rp < g.nzd) && CX(cp >= 0 && cp < g.nxd))) continue;
             const i64 ip = (i64)rp * g.nxd + cp;
             T *acc = a.gacc + (i64)s * a.nacc * g.np + ip;
             const i64 hs = (i64)s * a.nt_max;
@@ -1036,7 +1099,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
                     gl = ezm * Ax + exm * Az;
                     gm = T(2) * esm * Mes;
                 }
-                if (!INT && g.fs && r == 0 && c <= nx - 2) {
+                if (!IZ && g.fs && r == 0 && c <= nx - 2) {
                     // sigma_zz = 0 row: Dz_V = L_ratio Dx_U dz/dx; row 0 has no z-PML
                     const T tt = g.dtdz * eza0 * a.dxu0[(hs + it) * nx + c] * g.dz / g.dx;
                     gl += -tt / l2m;
@@ -1068,8 +1131,8 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
                 const T *Bi = a.Bimg + g.origin;
                 const int cpl = c + 1 == nx ? 0 : c + 1, cmi = c == 0 ? nx - 1 : c - 1;
                 const int rpl = r + 1 == nz ? 0 : r + 1, rmi = r == 0 ? nz - 1 : r - 1;
-                const i64 qxp = (i64)r * g.ld + cpl, qxm = (i64)r * g.ld + cmi;
-                const i64 qzp = (i64)rpl * g.ld + c, qzm = (i64)rmi * g.ld + c;
+                const int qxp = r * ld + cpl, qxm = r * ld + cmi;
+                const int qzp = rpl * ld + c, qzm = rmi * ld + c;
                 const T i2dx = T(2) * g.dx, i2dz = T(2) * g.dz;
                 const T dxBU = (Bi[qxp] * Ua[qxp] - Bi[qxm] * Ua[qxm]) / i2dx;
                 const T dzBV = (Bi[qzp] * Va[qzp] - Bi[qzm] * Va[qzm]) / i2dz;
@@ -1100,8 +1163,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
             }
         }
     };
-    if (inter) body(std::true_type{});
-    else body(std::false_type{});
+    STFWI_DISPATCH(inx, inz, body);
 }
 
 

@@ -194,3 +194,44 @@ __global__ void k_sos_traces(SosArgs<T> a)
 }
 
 }  // namespace stfwi
+
+namespace stfwi {
+
+// Zhou imaging of given history cubes (nt x nz x nx each) on the padded grid:
+// g_lambda = -dt sum_t (exx+ezz) div(B v_a), g_mu = -dt sum_t [2 exx d_x(B U_a)
+// + 2 ezz d_z(B V_a) + exz (d_z(B U_a) + d_x(B V_a))], g_rho_lame =
+// -dt sum_{t=1}^{nt-2} v(t) . (v_a(t+1) - v_a(t-1)) / 2dt, centred periodic
+// differences (fwi/imaging.py:125-152).  One thread per cell walks t.
+template <typename T>
+__global__ void k_zhou_cubes(const T *__restrict__ ex, const T *__restrict__ ez, const T *__restrict__ es,
+                             const T *__restrict__ ua, const T *__restrict__ va, const T *__restrict__ B,
+                             const T *__restrict__ vx, const T *__restrict__ vz, int nt, int nz, int nx, T dx,
+                             T dz, T dt, T *out)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+    if (c >= nx || r >= nz) return;
+    const long long N = (long long)nz * nx;
+    const int cp = c + 1 == nx ? 0 : c + 1, cm = c == 0 ? nx - 1 : c - 1;
+    const int rp = r + 1 == nz ? 0 : r + 1, rm = r == 0 ? nz - 1 : r - 1;
+    const long long q = (long long)r * nx + c, qxp = (long long)r * nx + cp, qxm = (long long)r * nx + cm;
+    const long long qzp = (long long)rp * nx + c, qzm = (long long)rm * nx + c;
+    const T bxp = B[qxp], bxm = B[qxm], bzp = B[qzp], bzm = B[qzm];
+    const T i2dx = T(2) * dx, i2dz = T(2) * dz, i2dt = T(2) * dt;
+    T gl = 0, gm = 0, gr = 0;
+    for (int t = 0; t < nt; ++t) {
+        const long long o = (long long)t * N;
+        const T dxBU = (bxp * ua[o + qxp] - bxm * ua[o + qxm]) / i2dx;
+        const T dzBV = (bzp * va[o + qzp] - bzm * va[o + qzm]) / i2dz;
+        const T dzBU = (bzp * ua[o + qzp] - bzm * ua[o + qzm]) / i2dz;
+        const T dxBV = (bxp * va[o + qxp] - bxm * va[o + qxm]) / i2dx;
+        gl += (ex[o + q] + ez[o + q]) * (dxBU + dzBV);
+        gm += T(2) * ex[o + q] * dxBU + T(2) * ez[o + q] * dzBV + es[o + q] * (dzBU + dxBV);
+        if (vx && t >= 1 && t <= nt - 2)
+            gr += vx[o + q] * ((ua[o + N + q] - ua[o - N + q]) / i2dt) + vz[o + q] * ((va[o + N + q] - va[o - N + q]) / i2dt);
+    }
+    out[q] = -dt * gl;
+    out[N + q] = -dt * gm;
+    out[2 * N + q] = -dt * gr;
+}
+
+}  // namespace stfwi

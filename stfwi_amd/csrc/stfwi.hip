@@ -755,3 +755,24 @@ extern "C" int stfwi_sosfilt(void *x, int nt, int ntr, const double *sos, int n_
     return dtype == STFWI_F64 ? sos_t<double>(x, nt, ntr, sos, n_sos, n_taper, transpose, (hipStream_t)stream)
                               : sos_t<float>(x, nt, ntr, sos, n_sos, n_taper, transpose, (hipStream_t)stream);
 }
+
+extern "C" int stfwi_zhou_imaging(const void *ex, const void *ez, const void *es, const void *ua, const void *va,
+                                  const void *b_full, const void *vx, const void *vz, int nt, int nz, int nx,
+                                  double dx, double dz, double dt, int dtype, void *out, void *stream)
+{
+    if (!ex || !ez || !es || !ua || !va || !b_full || !out || nt < 1 || nz < 1 || nx < 1 || (!vx) != (!vz))
+        return fail(STFWI_ERR_INVALID, "bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    dim3 grid((nx + 127) / 128, nz);
+    if (dtype == STFWI_F64)
+        hipLaunchKernelGGL(k_zhou_cubes<double>, grid, dim3(128), 0, st, (const double *)ex, (const double *)ez,
+                           (const double *)es, (const double *)ua, (const double *)va, (const double *)b_full,
+                           (const double *)vx, (const double *)vz, nt, nz, nx, dx, dz, dt, (double *)out);
+    else
+        hipLaunchKernelGGL(k_zhou_cubes<float>, grid, dim3(128), 0, st, (const float *)ex, (const float *)ez,
+                           (const float *)es, (const float *)ua, (const float *)va, (const float *)b_full,
+                           (const float *)vx, (const float *)vz, nt, nz, nx, (float)dx, (float)dz, (float)dt,
+                           (float *)out);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
