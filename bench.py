@@ -57,7 +57,7 @@ def survey(seed=0):
     from stfwi_amd.models import layered_model, smooth_initial, ricker
     vs_t, vp_t, rho_t = layered_model(NZ, NX, seed=seed)
     vs0, vp0, rho0 = smooth_initial(vs_t, vp_t, rho_t, 10.0)
-    src_x = 10 + 20 * np.arange(NSHOT)
+    src_x = (10 + 20 * np.arange(NSHOT)) % NX
     src_z = np.full(NSHOT, 2)
     rec_x = np.arange(NREC)
     rec_z = np.zeros(NREC, int)
@@ -108,6 +108,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--imaging", default="exact", choices=["exact", "zhou"])
+    ap.add_argument("--shots", type=int, default=NSHOT,
+                    help="shots per gradient (the workload is 25; other values are for experiments)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="after one forward, time each time-step kernel (20 launches) and a "
                          "256 MiB device copy; for hardware-counter runs")
@@ -121,6 +123,8 @@ def main():
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    global NSHOT
+    NSHOT = args.shots
     sv = survey(0)
     f32 = torch.float32
     plan = Plan(NZ, NX, DX, DZ, DT, FC, PAD, fd_order=FD, free_surface=True, dtype=f32, n_shots_max=NSHOT,

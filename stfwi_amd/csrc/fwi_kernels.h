@@ -81,7 +81,7 @@ __device__ __forceinline__ int zstrip(const Geo<T> &g, int r, int c)
 template <typename T>
 __device__ __forceinline__ T pml_fwd(T D, T *psi, const T *prof, int j, int n)
 {
-    const T pn = prof[n + j] * (*psi) + prof[2 * n + j] * D;   // psi' = b psi + a D
+    const T pn = fma(prof[n + j], *psi, prof[2 * n + j] * D);   // psi' = b psi + a D
     *psi = pn;
     return D / prof[j] + pn;                                   // D' = D / kappa + psi'
 }
@@ -89,7 +89,7 @@ __device__ __forceinline__ T pml_fwd(T D, T *psi, const T *prof, int j, int n)
 template <typename T>
 __device__ __forceinline__ T pmlT_val(T G, T psi_old, const T *prof, int j, int n)
 {
-    return prof[3 * n + j] * G + prof[2 * n + j] * psi_old;    // (1/kappa + a) G + a psi
+    return fma(prof[3 * n + j], G, prof[2 * n + j] * psi_old); // (1/kappa + a) G + a psi
 }
 
 template <typename T>
@@ -419,8 +419,8 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
                 const int rp = r - g.z0, cp = c - g.pad;
                 if (CZ(rp >= 0 && rp < g.nzd) && CX(cp >= 0 && cp < g.nxd)) {
                     T *h = a.hist2 + ((i64)s * a.nt_max + it) * 2 * g.np + (i64)rp * g.nxd + cp;
-                    h[0] = g.dtdx * D1 + g.dtdx * D2 + g.dtdz * D3;
-                    h[g.np] = g.dtdx * D4 + g.dtdz * D5 + g.dtdz * D6 + sv;
+                    h[0] = fma(g.dtdz, D3, fma(g.dtdx, D2, g.dtdx * D1));
+                    h[g.np] = fma(g.dtdz, D6, fma(g.dtdz, D5, g.dtdx * D4)) + sv;
                 }
             }
         }
@@ -549,7 +549,7 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
                 }
                 if (!IZ && iz >= 0) dzu = pml_fwd(dzu, pz + 4 * a.psiz_n, a.profz, r, nz);
                 if (!IX && ix >= 0) dxv = pml_fwd(dxv, px + 4 * a.psix_n, a.profx, c, nx);
-                esn = esn + T(0.5) * (g.dtdz * dzu + g.dtdx * dxv);
+                esn = esn + T(0.5) * fma(g.dtdz, dzu, g.dtdx * dxv);
                 es[q] = esn;
             }
             if (!IZ && a.dxu0 && r == 0) a.dxu0[((i64)s * a.nt_max + it) * nx + c] = dxu;
@@ -1095,8 +1095,8 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
                 if (it > 0) {
                     const T *h = a.hist + (hs + it - 1) * 3 * g.np + ip;
                     const T exm = h[0], ezm = h[g.np], esm = h[2 * g.np];
-                    gl2m = exm * Ax + ezm * Az;
-                    gl = ezm * Ax + exm * Az;
+                    gl2m = fma(exm, Ax, ezm * Az);
+                    gl = fma(ezm, Ax, exm * Az);
                     gm = T(2) * esm * Mes;
                 }
                 if (!IZ && g.fs && r == 0 && c <= nx - 2) {
@@ -1114,7 +1114,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
                 }
                 if (a.gparams == 3) {
                     const T *h2 = a.hist2 + (hs + it) * 2 * g.np + ip;
-                    gb = Ua[q] * h2[0] + Va[q] * h2[g.np];
+                    gb = fma(Ua[q], h2[0], Va[q] * h2[g.np]);
                     acc[0] += gl2m;
                     acc[g.np] += gl;
                     acc[2 * g.np] += gm;
@@ -1134,12 +1134,12 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
                 const int qxp = r * ld + cpl, qxm = r * ld + cmi;
                 const int qzp = rpl * ld + c, qzm = rmi * ld + c;
                 const T i2dx = T(2) * g.dx, i2dz = T(2) * g.dz;
-                const T dxBU = (Bi[qxp] * Ua[qxp] - Bi[qxm] * Ua[qxm]) / i2dx;
-                const T dzBV = (Bi[qzp] * Va[qzp] - Bi[qzm] * Va[qzm]) / i2dz;
-                const T dzBU = (Bi[qzp] * Ua[qzp] - Bi[qzm] * Ua[qzm]) / i2dz;
-                const T dxBV = (Bi[qxp] * Va[qxp] - Bi[qxm] * Va[qxm]) / i2dx;
+                const T dxBU = fma(Bi[qxp], Ua[qxp], -(Bi[qxm] * Ua[qxm])) / i2dx;
+                const T dzBV = fma(Bi[qzp], Va[qzp], -(Bi[qzm] * Va[qzm])) / i2dz;
+                const T dzBU = fma(Bi[qzp], Ua[qzp], -(Bi[qzm] * Ua[qzm])) / i2dz;
+                const T dxBV = fma(Bi[qxp], Va[qxp], -(Bi[qxm] * Va[qxm])) / i2dx;
                 const T glam = (ex + ez) * (dxBU + dzBV);
-                const T gmu = T(2) * ex * dxBU + T(2) * ez * dzBV + es * (dzBU + dxBV);
+                const T gmu = fma(es, dzBU + dxBV, fma(T(2) * ez, dzBV, T(2) * ex * dxBU));
                 if (a.gparams == 3) {
                     // density term: U_a(t) pairs with v(t-1) - v(t+1) (summation by
                     // parts of sum_{t=1}^{nt-2} v(t) (U_a(t+1) - U_a(t-1)) / 2dt)

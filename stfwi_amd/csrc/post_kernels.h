@@ -118,11 +118,14 @@ __global__ void k_eprecond3(const T *__restrict__ ws, int n_shots, int nz_t, int
     }
     const double add = eps * red[0];
     __syncthreads();
+    // We = |sqrt(Ws) (...)| + eps max(We)
+    for (long long i = threadIdx.x; i < NT; i += blockDim.x) we_pad[i] = (T)((double)we_pad[i] + add);
+    __syncthreads();
     const long long N = (long long)nzd * nxd;
     double pm = 0;
     for (long long i = threadIdx.x; i < N; i += blockDim.x) {
         const long long r = i / nxd, c = i % nxd;
-        const double p = 1.0 / ((double)we_pad[(r + z0) * nx_t + c + pad] + add);
+        const double p = 1.0 / (double)we_pad[(r + z0) * nx_t + c + pad];
         out[i] = (T)p;
         pm = fmax(pm, p);
     }

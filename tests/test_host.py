@@ -1,0 +1,122 @@
+"""CPU-side checks: the C-ABI library and its header agree, argument errors
+are reported without touching a device, and the host helpers behave."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "stfwi.h")
+
+
+def _declared():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(stfwi_[a-z0-9_]+)\s*\(", src)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from stfwi_amd import _abi
+    if not os.path.exists(_abi.LIB_PATH):
+        from stfwi_amd import build
+        build.build()
+    return _abi.lib()
+
+
+def test_library_exports_every_declared_symbol(lib):
+    names = _declared()
+    assert len(names) >= 18
+    out = subprocess.run(["nm", "-D", "--defined-only", os.path.join(ROOT, "stfwi_amd", "libstfwi.so")],
+                         capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (stfwi_[a-z0-9_]+)", out))
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    # and the ctypes table binds exactly those
+    from stfwi_amd._abi import SIGNATURES
+    assert set(SIGNATURES) <= set(names)
+    assert lib.stfwi_abi_version() == 1
+
+
+def _cfg(**kw):
+    from stfwi_amd import _abi
+    base = dict(nz_dom=20, nx_dom=30, pad=6, free_surface=1, fd_order=8, dx=10.0, dz=10.0, dt=1e-3, fc=10.0,
+                dtype=_abi.F32, n_shots_max=1, nt_max=10, nrec_max=3, imaging=0, grad_params=1, store_history=0)
+    base.update(kw)
+    return _abi.Config(**base)
+
+
+@pytest.mark.parametrize("bad,code", [
+    (dict(fd_order=6), -2), (dict(fd_order=3), -2), (dict(dtype=7), -1), (dict(nt_max=0), -1),
+    (dict(n_shots_max=0), -1), (dict(imaging=5), -1), (dict(grad_params=0), -1), (dict(dt=0.0), -1),
+    (dict(nz_dom=1, nx_dom=1, pad=1), -1),
+])
+def test_plan_create_rejects_bad_config_without_device(lib, bad, code):
+    h = C.c_void_p()
+    rc = lib.stfwi_plan_create(C.byref(_cfg(**bad)), C.byref(h))
+    assert rc == code
+    assert h.value is None
+    assert lib.stfwi_last_error().decode()
+
+
+def test_status_maps_to_python_exceptions(lib):
+    from stfwi_amd import _abi
+    h = C.c_void_p()
+    with pytest.raises(NotImplementedError):
+        _abi.check(lib.stfwi_plan_create(C.byref(_cfg(fd_order=6)), C.byref(h)))
+    with pytest.raises(ValueError):
+        _abi.check(lib.stfwi_plan_create(C.byref(_cfg(nt_max=0)), C.byref(h)))
+    assert lib.stfwi_plan_destroy(None) == 0
+
+
+def test_null_arguments_are_rejected(lib):
+    assert lib.stfwi_set_model(None, None, None, None, None, None) == -1
+    assert lib.stfwi_forward(None, None, 0, None, 0, None, 0, None) == -1
+    assert lib.stfwi_adjoint(None, None, None, None, 0, None) == -1
+    assert lib.stfwi_smooth2d(None, None, None, 4, 4, 1.0, 0, None) == -1
+    assert lib.stfwi_sosfilt(None, 10, 1, None, 0, 0, 0, 0, None) == -1
+
+
+def test_algorithmic_bytes_match_published_targets():
+    """The 152x500 / nt=3000 / FP32 shot-gradient moves 40.54 GB, the 100%
+    roofline is 197.3 shot-gradients/s (FP64: 81.08 GB)."""
+    import bench
+    tot32, _, _ = bench.algorithmic_bytes_per_shot_gradient(4, 1)
+    tot64, _, _ = bench.algorithmic_bytes_per_shot_gradient(8, 1)
+    assert abs(tot32 / 1e9 - 40.54) < 0.01
+    assert abs(tot64 / 1e9 - 81.08) < 0.01
+    assert abs(8e12 / tot32 - 197.3) < 0.1
+
+
+def test_layered_model_generator():
+    from stfwi_amd.models import layered_model, smooth_initial, anomaly_model, ricker
+    a = layered_model(152, 500, seed=3)
+    b = layered_model(152, 500, seed=3)
+    c = layered_model(152, 500, seed=4)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+    assert not np.array_equal(a[1], c[1])
+    vs, vp, rho = a
+    assert vp.min() >= 1600 * 0.9 - 1e-9 and vp.max() <= 4700 * 1.1 + 1e-9
+    ratio = vp / vs
+    assert ratio.min() >= 1.6 - 1e-9 and ratio.max() <= 2.0 + 1e-9
+    np.testing.assert_allclose(rho, 310.0 * vp ** 0.25)
+    # velocity increases with depth on average
+    assert vp[-10:].mean() > vp[:10].mean()
+    s = smooth_initial(vs, vp, rho, 10.0)
+    assert all(x.shape == (152, 500) for x in s)
+    assert np.abs(np.diff(s[1], axis=0)).max() < np.abs(np.diff(vp, axis=0)).max()
+    vs2, vp2, rho2 = anomaly_model(64, 128, seed=0)
+    assert vs2.max() > 1730 * 1.09 and np.all(vp2 == 3000.0)
+    w = ricker(100, 1e-3, 25.0)
+    assert np.argmax(w) == round(1.5 / 25.0 / 1e-3)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
