@@ -102,6 +102,7 @@ def cpu_baseline(sv, budget_s=15.0):
 
 
 def main():
+    global NSHOT
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
@@ -110,6 +111,7 @@ def main():
     ap.add_argument("--imaging", default="exact", choices=["exact", "zhou"])
     ap.add_argument("--shots", type=int, default=NSHOT,
                     help="shots per gradient (the workload is 25; other values are for experiments)")
+    ap.add_argument("--groups", type=int, default=2, help="concurrent stream groups of shots (1-4)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="after one forward, time each time-step kernel (20 launches) and a "
                          "256 MiB device copy; for hardware-counter runs")
@@ -123,7 +125,6 @@ def main():
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
-    global NSHOT
     NSHOT = args.shots
     sv = survey(0)
     f32 = torch.float32
@@ -141,6 +142,7 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"kernels_only": res, "tiles": tiles, "copy_bytes": a.numel() * 4}))
         return
+    plan.set_groups(args.groups)
     # observed data on the true model
     plan.set_model(*sv["true"])
     plan.set_shots(sv["src_x"], sv["src_z"], "moment", NT, sv["rec_x"], sv["rec_z"], sv["wav"])
@@ -191,7 +193,7 @@ def main():
            "config": {"workload": "25-shot gradient, 152x500 fd8 free surface pad 20, nt=3000, "
                                   "500 receivers, moment sources, exact imaging, Vs",
                       "shots": NSHOT, "nt": NT, "grid": [NZ, NX], "imaging": args.imaging},
-           "roofline": kern, "kernel_us": per_kernel, "tiles": tiles, "autotune_us": tune, "path_roofline": path, "cpu_baseline": cpu,
+           "roofline": kern, "kernel_us": per_kernel, "tiles": tiles, "groups": args.groups, "autotune_us": tune, "path_roofline": path, "cpu_baseline": cpu,
            "finite": ok, "loss": float(loss.double().sum())}
     print(json.dumps(out))
 

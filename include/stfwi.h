@@ -189,15 +189,20 @@ int stfwi_time_kernel(stfwi_plan *plan, int which, int reps, void *stream, doubl
 
 /* Tile-shape autotuner (no reference counterpart): for each of the four
  * time-step kernels time `reps` launches at every compiled tile height
- * (8/16/32 rows of 64 columns in FP32, 8 in FP64) on the plan's shots and
- * keep the fastest.  tz_out (4 ints) and us_out (4 x 3 average microseconds,
- * 0 where not compiled) may be NULL.  Host call, synchronous; overwrites the
+ * (4/8/16/32 rows of 64 columns in FP32, 4/8 in FP64) on the plan's shots
+ * and keep the fastest.  tz_out (4 ints) and us_out (4 x 4 average
+ * microseconds, 0 where not compiled) may be NULL.  Host call, synchronous; overwrites the
  * wavefield state, so run it before stfwi_forward. */
 int stfwi_autotune(stfwi_plan *plan, int reps, void *stream, int *tz_out, double *us_out);
 
 /* Set the four tile heights explicitly (kernel order: forward velocity,
  * forward strain, adjoint strain, adjoint velocity). */
 int stfwi_set_tiles(stfwi_plan *plan, const int *tz);
+
+/* Split the shots into `groups` (1..4) contiguous groups whose time loops
+ * run concurrently on the plan's own streams (default priority), joined
+ * back into the caller's stream; results do not depend on the grouping. */
+int stfwi_set_groups(stfwi_plan *plan, int groups);
 
 #ifdef __cplusplus
 }

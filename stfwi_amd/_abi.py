@@ -34,6 +34,7 @@ SIGNATURES = {
     "stfwi_time_kernel": (C.c_int, [_VP, C.c_int, C.c_int, _VP, C.POINTER(C.c_double)]),
     "stfwi_autotune": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP]),
     "stfwi_set_tiles": (C.c_int, [_VP, _VP]),
+    "stfwi_set_groups": (C.c_int, [_VP, C.c_int]),
     "stfwi_smooth2d": (C.c_int, [_VP, _VP, _VP, C.c_int, C.c_int, C.c_double, C.c_int, _VP]),
     "stfwi_illumination": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                      C.c_double, C.c_int, _VP, _VP]),

@@ -211,6 +211,7 @@ struct FwdArgs {
     T *snap;                         // [shot][nsnap][nt_max][nz*nx]
     int snap_mask, nsnap;
     int tiles_x, tiles_z, ntiles;
+    int shot0;                         // first shot of this launch's group
 };
 
 #define TILE_X 64
@@ -279,6 +280,7 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
     const Geo<T> &g = a.g;
     int bx, s;
     logical_block(bx, s);
+    s += a.shot0;
     const ShotDev sh = a.shots[s];
     if (bx >= a.ntiles) {
         const int j = (bx - a.ntiles) * 256 + threadIdx.y * TILE_X + threadIdx.x;
@@ -356,8 +358,19 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
             }
         }
     }
-    __syncthreads();
+    // owned values, loaded ahead of the barrier
     const int c = c0 + lx;
+    T u0[RPT], v0[RPT], b0[RPT];
+#pragma unroll
+    for (int rr = 0; rr < RPT; ++rr) {
+        const int r = r0 + ty + BLOCK_Y * rr;
+        const bool ok = r < nz && c < nx;
+        const int q = ok ? r * ld + c : 0;
+        u0[rr] = U[q];
+        v0[rr] = V[q];
+        b0[rr] = B[q];
+    }
+    __syncthreads();
     auto body = [&](auto tagx, auto tagz) {
         constexpr bool IX = decltype(tagx)::value, IZ = decltype(tagz)::value;
 #pragma unroll
@@ -405,9 +418,9 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
                     D6 = pml_fwd(D6, pz + 2 * a.psiz_n, a.profz, r, nz);
                 }
             }
-            const T b = B[q];
-            U[q] = U[q] + g.dtdx * b * D1 + g.dtdx * b * D2 + g.dtdz * b * D3;
-            T v = V[q] + g.dtdx * b * D4 + g.dtdz * b * D5 + g.dtdz * b * D6;
+            const T b = b0[rr];
+            U[q] = u0[rr] + g.dtdx * b * D1 + g.dtdx * b * D2 + g.dtdz * b * D3;
+            T v = v0[rr] + g.dtdx * b * D4 + g.dtdz * b * D5 + g.dtdz * b * D6;
             T sv = 0;
             if (sh.type == 0 && r == sh.sz && c == sh.sx) {
                 const T w = a.wav[(i64)s * a.nt_max + it];
@@ -436,6 +449,7 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
     const Geo<T> &g = a.g;
     int bx, s;
     logical_block(bx, s);
+    s += a.shot0;
     const ShotDev sh = a.shots[s];
     if (it >= sh.nt) return;
     if (bx >= a.ntiles) {
@@ -490,8 +504,17 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
             }
         }
     }
-    __syncthreads();
     const int c = c0 + lx;
+    T ex0[RPT], ez0[RPT], es0[RPT];
+#pragma unroll
+    for (int rr = 0; rr < RPT; ++rr) {
+        const int r = r0 + ty + BLOCK_Y * rr;
+        const int q = (r < nz && c < nx) ? r * ld + c : 0;
+        ex0[rr] = ex[q];
+        ez0[rr] = ez[q];
+        es0[rr] = es[q];
+    }
+    __syncthreads();
     auto body = [&](auto tagx, auto tagz) {
         constexpr bool IX = decltype(tagx)::value, IZ = decltype(tagz)::value;
 #pragma unroll
@@ -514,7 +537,7 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
             T *px = a.psix + s * a.psix_stride + ix;
             T *pz = a.psiz + s * a.psiz_stride + iz;
             if (!IX && ix >= 0) dxu = pml_fwd(dxu, px + 3 * a.psix_n, a.profx, c, nx);
-            T exn = CX(c >= 1) ? ex[q] + g.dtdx * dxu : ex[q];
+            T exn = CX(c >= 1) ? ex0[rr] + g.dtdx * dxu : ex0[rr];
             // ezz: D-z V on columns [0, nx-2]; free-surface row from the C-PML'd D-x U
             T dzv = 0;
             if (CX(c <= nx - 2)) {
@@ -528,7 +551,7 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
                 }
             }
             if (!IZ && iz >= 0) dzv = pml_fwd(dzv, pz + 3 * a.psiz_n, a.profz, r, nz);
-            T ezn = ez[q] + g.dtdz * dzv;
+            T ezn = ez0[rr] + g.dtdz * dzv;
             if (sh.type == 1 && r == sh.sz && c == sh.sx) {
                 const T lm = (a.L2M[g.origin + q] - T(2) * a.M[g.origin + q]) + a.M[g.origin + q];
                 const T stm = a.wav[(i64)s * a.nt_max + it] * g.dt / (T(2) * lm * g.dx * g.dz);
@@ -538,7 +561,7 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
             ex[q] = exn;
             ez[q] = ezn;
             // exz on [0, nz-2] x [0, nx-2]
-            T esn = es[q];
+            T esn = es0[rr];
             if (CZ(r <= nz - 2) && CX(c <= nx - 2)) {
                 T dzu = 0, dxv = 0;
 #pragma unroll
@@ -614,6 +637,7 @@ struct AdjArgs {
     T *snap;
     int snap_mask, nsnap;
     int tiles_x, tiles_z, ntiles;
+    int shot0;                         // first shot of this launch's group
 };
 
 // receiver-adjoint injection at the start of reverse step it
@@ -621,7 +645,7 @@ struct AdjArgs {
 template <typename T>
 __global__ void k_adj_inject(AdjArgs<T> a, int it)
 {
-    const int s = blockIdx.z;
+    const int s = a.shot0 + blockIdx.z;
     const ShotDev sh = a.shots[s];
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
     if (it >= sh.nt || u >= sh.nuc) return;
@@ -696,6 +720,7 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
     const Geo<T> &g = a.g;
     int bx, s;
     logical_block(bx, s);
+    s += a.shot0;
     const ShotDev sh = a.shots[s];
     if (it >= sh.nt) return;
     constexpr int RPT = TZ / BLOCK_Y;
@@ -923,6 +948,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
     const Geo<T> &g = a.g;
     int bx, s;
     logical_block(bx, s);
+    s += a.shot0;
     const ShotDev sh = a.shots[s];
     if (it >= sh.nt) return;
     constexpr int RPT = TZ / BLOCK_Y;
@@ -1002,6 +1028,28 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
             }
         }
     }
+    // owned values and the history frame, loaded ahead of the barrier
+    T l2m0[RPT], mu0[RPT], exa0v[RPT], eza0v[RPT], esa0v[RPT], h0[RPT], h1[RPT], h2v[RPT];
+    const i64 hs = (i64)s * a.nt_max;
+    const int hframe = a.imaging == 2 ? it - 1 : it;
+#pragma unroll
+    for (int rr = 0; rr < RPT; ++rr) {
+        const int r = r0 + ty + BLOCK_Y * rr;
+        const int q = (r < nz && c < nx) ? r * ld + c : 0;
+        l2m0[rr] = a.L2M[g.origin + q];
+        mu0[rr] = a.M[g.origin + q];
+        exa0v[rr] = exa[q];
+        eza0v[rr] = eza[q];
+        esa0v[rr] = esa[q];
+        const int rp = r - g.z0, cp = c - g.pad;
+        h0[rr] = h1[rr] = h2v[rr] = T(0);
+        if (a.imaging && hframe >= 0 && rp >= 0 && rp < g.nzd && cp >= 0 && cp < g.nxd) {
+            const T *h = a.hist + (hs + hframe) * 3 * g.np + (i64)rp * g.nxd + cp;
+            h0[rr] = h[0];
+            h1[rr] = h[g.np];
+            h2v[rr] = h[2 * g.np];
+        }
+    }
     __syncthreads();
     auto body = [&](auto tagx, auto tagz) {
         constexpr bool IX = decltype(tagx)::value, IZ = decltype(tagz)::value;
@@ -1048,12 +1096,12 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
             }
             if (!IZ && g.fs && r == 0 && c >= 1 && c <= nx - 2) Mes += T(2 * fdc<FD>(1)) * szMes[zc][lx];
 
-            const T l2m = a.L2M[g.origin + q], mu = a.M[g.origin + q];
+            const T l2m = l2m0[rr], mu = mu0[rr];
             const T lam = l2m - T(2) * mu;
-            const T exa0 = exa[q], eza0 = eza[q];
+            const T exa0 = exa0v[rr], eza0 = eza0v[rr];
             exa[q] = exa0 + l2m * Ax + lam * Az;
             eza[q] = eza0 + lam * Ax + l2m * Az;
-            esa[q] = esa[q] + T(2) * mu * Mes;
+            esa[q] = esa0v[rr] + T(2) * mu * Mes;
 
             if (!IX) {
                 const int ix = xstrip(g, r, c);
@@ -1087,14 +1135,12 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
             if (a.imaging == 0 || !(CZ(rp >= 0 && rp < g.nzd) && CX(cp >= 0 && cp < g.nxd))) continue;
             const i64 ip = (i64)rp * g.nxd + cp;
             T *acc = a.gacc + (i64)s * a.nacc * g.np + ip;
-            const i64 hs = (i64)s * a.nt_max;
             if (a.imaging == 2) {
                 // exact discrete gradient: model dependence of step it, paired with
                 // its carry-in strains (history frame it-1)
                 T gl2m = 0, gl = 0, gm = 0, gb = 0;
                 if (it > 0) {
-                    const T *h = a.hist + (hs + it - 1) * 3 * g.np + ip;
-                    const T exm = h[0], ezm = h[g.np], esm = h[2 * g.np];
+                    const T exm = h0[rr], ezm = h1[rr], esm = h2v[rr];
                     gl2m = fma(exm, Ax, ezm * Az);
                     gl = fma(ezm, Ax, exm * Az);
                     gm = T(2) * esm * Mes;
@@ -1126,8 +1172,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
             } else {
                 // Zhou imaging (fwi/imaging.py:125-152): forward frame it with the
                 // adjoint carry-out of step it; centred periodic differences of B v_a
-                const T *h = a.hist + (hs + it) * 3 * g.np + ip;
-                const T ex = h[0], ez = h[g.np], es = h[2 * g.np];
+                const T ex = h0[rr], ez = h1[rr], es = h2v[rr];
                 const T *Bi = a.Bimg + g.origin;
                 const int cpl = c + 1 == nx ? 0 : c + 1, cmi = c == 0 ? nx - 1 : c - 1;
                 const int rpl = r + 1 == nz ? 0 : r + 1, rmi = r == 0 ? nz - 1 : r - 1;

@@ -39,6 +39,8 @@ static int fail(int code, const char *fmt, ...)
             return fail(STFWI_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(e_));         \
     } while (0)
 
+#define STFWI_MAX_GROUPS 4
+
 struct stfwi_plan {
     stfwi_config cfg;
     int esz = 4;
@@ -56,6 +58,11 @@ struct stfwi_plan {
     void *gacc = nullptr;
     int hist2_mode = 0;
     int tz[4] = {16, 16, 16, 16};   // tile height per time-step kernel (autotuned)
+    // shots are split into `groups` contiguous groups whose time loops run on
+    // their own streams, so one group's launches fill the other's tails
+    int groups = 1;
+    hipStream_t gstream[STFWI_MAX_GROUPS] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[STFWI_MAX_GROUPS] = {};
     ShotDev *shots_d = nullptr;
     i64 *rec_cell_d = nullptr, *uc_cell_d = nullptr;
     int *uc_ptr_d = nullptr, *uc_rec_d = nullptr;
@@ -121,6 +128,11 @@ extern "C" int stfwi_plan_destroy(stfwi_plan *plan)
 {
     if (!plan) return 0;
     for (void *a : plan->allocs) (void)hipFree(a);
+    for (int k = 0; k < STFWI_MAX_GROUPS; ++k) {
+        if (plan->gstream[k]) (void)hipStreamDestroy(plan->gstream[k]);
+        if (plan->ev_join[k]) (void)hipEventDestroy(plan->ev_join[k]);
+    }
+    if (plan->ev_fork) (void)hipEventDestroy(plan->ev_fork);
     delete plan;
     return 0;
 }
@@ -191,6 +203,12 @@ extern "C" int stfwi_plan_create(const stfwi_config *cfg, stfwi_plan **out)
         g_err = e;
         return STFWI_ERR_OOM;
     }
+    // streams of the shot groups (default priority) and their fork/join events
+    for (int k = 0; k < STFWI_MAX_GROUPS; ++k) {
+        HIP_TRY(hipStreamCreateWithFlags(&p->gstream[k], hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&p->ev_join[k], hipEventDisableTiming));
+    }
+    HIP_TRY(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
     // margins of the moduli buffers must be zero
     hipStream_t st = nullptr;
     if (c.dtype == STFWI_F64) fill<double>(p->mod, 4 * p->fstride + p->nx, st);
@@ -297,12 +315,12 @@ extern "C" int stfwi_set_shots(stfwi_plan *p, const stfwi_shot *shots, int n_sho
 }
 
 
-// tile heights compiled per precision: 8/16/32 rows in FP32, 8 in FP64
+// tile heights compiled per precision: 4/8/16/32 rows in FP32, 4/8 in FP64
 // (larger FP64 tiles would exceed 64 KiB of LDS in the forward kernel)
 template <typename T>
 static bool tz_supported(int tz)
 {
-    return std::is_same<T, float>::value ? (tz == 8 || tz == 16 || tz == 32) : tz == 8;
+    return std::is_same<T, float>::value ? (tz == 4 || tz == 8 || tz == 16 || tz == 32) : (tz == 4 || tz == 8);
 }
 
 // kernel ids: 0 forward velocity, 1 forward strain, 2 adjoint strain,
@@ -321,9 +339,11 @@ static void launch_fwd(int which, int tz, FwdArgs<T> a, int rblk, int S, hipStre
     if constexpr (std::is_same<T, float>::value) {
         if (tz == 32) STFWI_LF(32);
         else if (tz == 16) STFWI_LF(16);
+        else if (tz == 4) STFWI_LF(4);
         else STFWI_LF(8);
     } else {
-        STFWI_LF(8);
+        if (tz == 4) STFWI_LF(4);
+        else STFWI_LF(8);
     }
 #undef STFWI_LF
 }
@@ -342,11 +362,46 @@ static void launch_adj(int which, int tz, AdjArgs<T> a, int S, hipStream_t st, i
     if constexpr (std::is_same<T, float>::value) {
         if (tz == 32) STFWI_LA(32);
         else if (tz == 16) STFWI_LA(16);
+        else if (tz == 4) STFWI_LA(4);
         else STFWI_LA(8);
     } else {
-        STFWI_LA(8);
+        if (tz == 4) STFWI_LA(4);
+        else STFWI_LA(8);
     }
 #undef STFWI_LA
+}
+
+// contiguous shot groups; returns the group count, g0[k]..g0[k+1] bound group k
+static int group_bounds(const stfwi_plan *p, int *g0)
+{
+    const int G = std::max(1, std::min(p->groups, p->n_shots));
+    for (int k = 0; k <= G; ++k) g0[k] = (int)((long long)p->n_shots * k / G);
+    return G;
+}
+
+static hipError_t fork_groups(stfwi_plan *p, int G, hipStream_t st, hipStream_t *gs)
+{
+    if (G == 1) {
+        gs[0] = st;
+        return hipSuccess;
+    }
+    hipError_t e = hipEventRecord(p->ev_fork, st);
+    for (int k = 0; k < G && e == hipSuccess; ++k) {
+        gs[k] = p->gstream[k];
+        e = hipStreamWaitEvent(gs[k], p->ev_fork, 0);
+    }
+    return e;
+}
+
+static hipError_t join_groups(stfwi_plan *p, int G, hipStream_t st)
+{
+    if (G == 1) return hipSuccess;
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < G && e == hipSuccess; ++k) {
+        e = hipEventRecord(p->ev_join[k], p->gstream[k]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, p->ev_join[k], 0);
+    }
+    return e;
 }
 
 template <typename T>
@@ -415,13 +470,24 @@ static int forward_t(stfwi_plan *p, void *rec, int rec_mask, void *illum, int il
     fill<T>(p->psix_f, (i64)S * 5 * p->psix_len, st);
     fill<T>(p->psiz_f, (i64)S * 5 * p->psiz_len, st);
     const int rblk = rec ? (p->max_nrec + 255) / 256 : 0;
-    for (int it = 0; it < p->nt_run; ++it) {
-        launch_fwd<T, FD>(0, p->tz[0], a, rblk, S, st, it);
-        launch_fwd<T, FD>(1, p->tz[1], a, rblk, S, st, it);
-    }
+    int g0[STFWI_MAX_GROUPS + 1];
+    const int G = group_bounds(p, g0);
+    hipStream_t gs[STFWI_MAX_GROUPS];
+    HIP_TRY(fork_groups(p, G, st, gs));
+    for (int it = 0; it < p->nt_run; ++it)
+        for (int k = 0; k < G; ++k) {
+            a.shot0 = g0[k];
+            launch_fwd<T, FD>(0, p->tz[0], a, rblk, g0[k + 1] - g0[k], gs[k], it);
+            launch_fwd<T, FD>(1, p->tz[1], a, rblk, g0[k + 1] - g0[k], gs[k], it);
+        }
     // exx/ezz receivers of the last step
-    if (rec) launch_fwd<T, FD>(0, p->tz[0], a, rblk, S, st, p->nt_run);
+    if (rec)
+        for (int k = 0; k < G; ++k) {
+            a.shot0 = g0[k];
+            launch_fwd<T, FD>(0, p->tz[0], a, rblk, g0[k + 1] - g0[k], gs[k], p->nt_run);
+        }
     HIP_TRY(hipGetLastError());
+    HIP_TRY(join_groups(p, G, st));
     return 0;
 }
 
@@ -458,16 +524,24 @@ static int adjoint_t(stfwi_plan *p, const void *rec_adj, void *src_adj, void *sn
         fill<T>(p->psiz_a[k], (i64)S * 4 * p->psiz_len, st);
     }
     if (a.imaging) fill<T>(p->gacc, (i64)S * p->nacc * p->np, st);
-    dim3 gi((p->max_nuc + 63) / 64, 1, S);
+    int g0[STFWI_MAX_GROUPS + 1];
+    const int G = group_bounds(p, g0);
+    hipStream_t gs[STFWI_MAX_GROUPS];
+    HIP_TRY(fork_groups(p, G, st, gs));
     for (int it = p->nt_run - 1; it >= 0; --it) {
         const int cur = it & 1, prev = cur ^ 1;
         a.psix_o = (const T *)p->psix_a[prev]; a.psiz_o = (const T *)p->psiz_a[prev];
         a.psix_n = (T *)p->psix_a[cur]; a.psiz_n = (T *)p->psiz_a[cur];
-        hipLaunchKernelGGL(k_adj_inject<T>, gi, dim3(64), 0, st, a, it);
-        launch_adj<T, FD>(2, p->tz[2], a, S, st, it);
-        launch_adj<T, FD>(3, p->tz[3], a, S, st, it);
+        for (int k = 0; k < G; ++k) {
+            const int ns = g0[k + 1] - g0[k];
+            a.shot0 = g0[k];
+            hipLaunchKernelGGL(k_adj_inject<T>, dim3((p->max_nuc + 63) / 64, 1, ns), dim3(64), 0, gs[k], a, it);
+            launch_adj<T, FD>(2, p->tz[2], a, ns, gs[k], it);
+            launch_adj<T, FD>(3, p->tz[3], a, ns, gs[k], it);
+        }
     }
     HIP_TRY(hipGetLastError());
+    HIP_TRY(join_groups(p, G, st));
     return 0;
 }
 
@@ -612,19 +686,19 @@ extern "C" int stfwi_autotune(stfwi_plan *p, int reps, void *stream, int *tz_out
 {
     if (!p || reps < 1) return fail(STFWI_ERR_INVALID, "bad argument");
     if (!p->have_model || !p->n_shots) return fail(STFWI_ERR_STATE, "model or shots not set");
-    const int cand[3] = {8, 16, 32};
+    const int cand[4] = {4, 8, 16, 32};
     const bool f64 = p->cfg.dtype == STFWI_F64;
     for (int which = 0; which < 4; ++which) {
         double best = 1e300;
         int best_tz = p->tz[which];
-        for (int ci = 0; ci < 3; ++ci) {
+        for (int ci = 0; ci < 4; ++ci) {
             const int tz = cand[ci];
             if (f64 ? !tz_supported<double>(tz) : !tz_supported<float>(tz)) continue;
             double ms = 0;
             int rc = time_kernel(p, which, tz, 2, (hipStream_t)stream, &ms);   // warm-up
             if (!rc) rc = time_kernel(p, which, tz, reps, (hipStream_t)stream, &ms);
             if (rc) return rc;
-            if (us_out) us_out[which * 3 + ci] = ms * 1e3 / reps;
+            if (us_out) us_out[which * 4 + ci] = ms * 1e3 / reps;
             if (ms < best) {
                 best = ms;
                 best_tz = tz;
@@ -636,6 +710,14 @@ extern "C" int stfwi_autotune(stfwi_plan *p, int reps, void *stream, int *tz_out
     // the timing launches ran on the live buffers
     p->fwd_done = false;
     p->adj_done = false;
+    return 0;
+}
+
+extern "C" int stfwi_set_groups(stfwi_plan *p, int groups)
+{
+    if (!p || groups < 1 || groups > STFWI_MAX_GROUPS)
+        return fail(STFWI_ERR_INVALID, "groups=%d outside [1, %d]", groups, STFWI_MAX_GROUPS);
+    p->groups = groups;
     return 0;
 }
 

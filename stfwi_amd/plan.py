@@ -218,12 +218,16 @@ class Plan:
         compiled variant on this plan's shots (run before forward()).
         Returns (tile heights, {kernel: {tz: microseconds}})."""
         tz = (C.c_int * 4)()
-        us = (C.c_double * 12)()
+        us = (C.c_double * 16)()
         check(lib().stfwi_autotune(self._h, int(reps), _stream(), tz, us))
         names = ("fwd_vel", "fwd_strain", "adj_strain", "adj_vel")
-        table = {n: {h: us[3 * i + j] for j, h in enumerate((8, 16, 32)) if us[3 * i + j] > 0}
+        table = {n: {h: us[4 * i + j] for j, h in enumerate((4, 8, 16, 32)) if us[4 * i + j] > 0}
                  for i, n in enumerate(names)}
         return list(tz), table
+
+    def set_groups(self, groups):
+        """Run the shots as `groups` concurrent stream groups (1..4)."""
+        check(lib().stfwi_set_groups(self._h, int(groups)))
 
     def set_tiles(self, tz):
         arr = (C.c_int * 4)(*[int(t) for t in tz])
