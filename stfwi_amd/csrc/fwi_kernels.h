@@ -77,6 +77,22 @@ __device__ __forceinline__ int zstrip(const Geo<T> &g, int r, int c)
     return -1;
 }
 
+// Global accesses of the time-step kernels: a block-uniform base pointer
+// plus a non-negative 32-bit element index, turned into a 32-bit byte offset
+// so the compiler emits base(SGPR) + offset(VGPR) addressing instead of
+// per-lane 64-bit pointer arithmetic.  Every index passed is >= 0.
+template <typename T>
+__device__ __forceinline__ T ldg(const T *base, int idx)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + (uint32_t)idx * (uint32_t)sizeof(T));
+}
+
+template <typename T>
+__device__ __forceinline__ void stg(T *base, int idx, T v)
+{
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (uint32_t)idx * (uint32_t)sizeof(T)) = v;
+}
+
 // profile layout per axis: kappa[n], b[n], a[n], q[n] with q = 1/kappa + a
 template <typename T>
 __device__ __forceinline__ T pml_fwd(T D, T *psi, const T *prof, int j, int n)
@@ -97,6 +113,30 @@ __device__ __forceinline__ T pmlT_psi(T G, T psi_old, const T *prof, int j, int 
 {
     return prof[n + j] * (G + psi_old);                        // b (G + psi)
 }
+
+// pmlT_val with the profile values and the memory already loaded
+template <typename T>
+__device__ __forceinline__ T pmlT_pre(T G, T psi_old, T q, T a)
+{
+    return fma(q, G, a * psi_old);
+}
+
+// C-PML inputs of the transposed recursion at one staged cell: the two old
+// memories and the (1/kappa + a, a) profile pair.  Cells outside the strip
+// (i < 0) read entry 0, which a select discards afterwards.
+template <typename T>
+struct PmlIn {
+    T m0, m1, q, a;
+    __device__ __forceinline__ void load(const T *mem0, const T *mem1, int i, const T *prof, int j, int n)
+    {
+        const int ic = i >= 0 ? i : 0;
+        const int jc = j < 0 ? 0 : (j >= n ? n - 1 : j);
+        m0 = ldg(mem0, ic);
+        m1 = ldg(mem1, ic);
+        q = ldg(prof + 3 * n, jc);
+        a = ldg(prof + 2 * n, jc);
+    }
+};
 
 // ---------------------------------------------------------------------------
 // model set-up (fwi/forward.py:193-238)
@@ -295,7 +335,7 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
     }
     if (it >= sh.nt) return;
     constexpr int RPT = TZ / BLOCK_Y;
-    __shared__ T sP1[TZ][TILE_X + 8], sP2[TZ][TILE_X + 8], sP3x[TZ][TILE_X + 8];
+    __shared__ T sP1[TZ + 1][TILE_X + 8], sP2[TZ + 1][TILE_X + 8], sP3x[TZ + 1][TILE_X + 8];
     __shared__ T sP3z[TZ + 8][TILE_X], sP4[TZ + 8][TILE_X], sP5[TZ + 8][TILE_X];
     constexpr int NK = FDC<FD>::NK;
     const int ld = g.ld, nz = g.nz, nx = g.nx;
@@ -312,63 +352,75 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
     const T *__restrict__ M = a.M + g.origin;
     const T *__restrict__ B = a.B + g.origin;
     // stress products: (lambda+2mu) exx, lambda ezz, 2 mu exz along x;
-    // 2 mu exz, (lambda+2mu) ezz, lambda exx along z
-    {
-        const int c = c0 + lx;
-#pragma unroll
-        for (int k = 0; k < (TZ + 8) / BLOCK_Y; ++k) {
-            const int zi = ty + BLOCK_Y * k, r = r0 - 4 + zi;
-            T p1 = 0, p2 = 0, p3 = 0, p4 = 0, p5 = 0;
-            if (r >= 0 && r < nz && c < nx) {
-                const int p = r * ld + c;
-                const T l2m = L2M[p], mu = M[p], exv = ex[p], ezv = ez[p];
-                const T lam = l2m - T(2) * mu;
-                p1 = l2m * exv;
-                p2 = lam * ezv;
-                p3 = T(2) * mu * es[p];
-                p4 = l2m * ezv;
-                p5 = lam * exv;
-            }
-            sP3z[zi][lx] = p3;
-            sP4[zi][lx] = p4;
-            sP5[zi][lx] = p5;
-            if (k >= 1 && k < TZ / BLOCK_Y + 1) {
-                sP1[zi - 4][lx + 4] = p1;
-                sP2[zi - 4][lx + 4] = p2;
-                sP3x[zi - 4][lx + 4] = p3;
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < (8 * TZ + 255) / 256; ++k) {
-            const int e = tid + 256 * k;
-            if ((8 * TZ) % 256 == 0 || e < 8 * TZ) {
-                const int zi = e >> 3, h = e & 7, xi = h < 4 ? h : h + TILE_X;
-                const int r = r0 + zi, cc = c0 - 4 + xi;
-                T p1 = 0, p2 = 0, p3 = 0;
-                if (r < nz && cc >= 0 && cc < nx) {
-                    const int p = r * ld + cc;
-                    const T l2m = L2M[p], mu = M[p];
-                    p1 = l2m * ex[p];
-                    p2 = (l2m - T(2) * mu) * ez[p];
-                    p3 = T(2) * mu * es[p];
-                }
-                sP1[zi][xi] = p1;
-                sP2[zi][xi] = p2;
-                sP3x[zi][xi] = p3;
-            }
-        }
-    }
-    // owned values, loaded ahead of the barrier
+    // 2 mu exz, (lambda+2mu) ezz, lambda exx along z.  Every load of the
+    // stage (z-slab, x-halo, owned values) is issued before the first use:
+    // cells outside the grid load cell 0 and are zeroed by a select.
+    constexpr int KZ = (TZ + 8) / BLOCK_Y, KX = (8 * TZ + 255) / 256;
     const int c = c0 + lx;
+    T zl[KZ], zm[KZ], zx[KZ], zz[KZ], zs[KZ];
+    bool zok[KZ];
+#pragma unroll
+    for (int k = 0; k < KZ; ++k) {
+        const int r = r0 - 4 + ty + BLOCK_Y * k;
+        zok[k] = r >= 0 && r < nz && c < nx;
+        const int p = zok[k] ? r * ld + c : 0;
+        zl[k] = ldg(L2M, p);
+        zm[k] = ldg(M, p);
+        zx[k] = ldg(ex, p);
+        zz[k] = ldg(ez, p);
+        zs[k] = ldg(es, p);
+    }
+    T xl[KX], xm[KX], xx[KX], xz[KX], xs[KX];
+    bool xok[KX];
+#pragma unroll
+    for (int k = 0; k < KX; ++k) {
+        const int e = tid + 256 * k;
+        const int zi = e >> 3, h = e & 7, xi = h < 4 ? h : h + TILE_X;
+        const int r = r0 + zi, cc = c0 - 4 + xi;
+        xok[k] = ((8 * TZ) % 256 == 0 || e < 8 * TZ) && r < nz && cc >= 0 && cc < nx;
+        const int p = xok[k] ? r * ld + cc : 0;
+        xl[k] = ldg(L2M, p);
+        xm[k] = ldg(M, p);
+        xx[k] = ldg(ex, p);
+        xz[k] = ldg(ez, p);
+        xs[k] = ldg(es, p);
+    }
     T u0[RPT], v0[RPT], b0[RPT];
 #pragma unroll
     for (int rr = 0; rr < RPT; ++rr) {
         const int r = r0 + ty + BLOCK_Y * rr;
-        const bool ok = r < nz && c < nx;
-        const int q = ok ? r * ld + c : 0;
-        u0[rr] = U[q];
-        v0[rr] = V[q];
-        b0[rr] = B[q];
+        const int q = (r < nz && c < nx) ? r * ld + c : 0;
+        u0[rr] = ldg((const T *)U, q);
+        v0[rr] = ldg((const T *)V, q);
+        b0[rr] = ldg(B, q);
+    }
+#pragma unroll
+    for (int k = 0; k < KZ; ++k) {
+        const int zi = ty + BLOCK_Y * k;
+        const T l2m = zl[k], mu = zm[k], exv = zx[k], ezv = zz[k];
+        const T lam = l2m - T(2) * mu;
+        const T p3 = zok[k] ? T(2) * mu * zs[k] : T(0);
+        sP3z[zi][lx] = p3;
+        sP4[zi][lx] = zok[k] ? l2m * ezv : T(0);
+        sP5[zi][lx] = zok[k] ? lam * exv : T(0);
+        if (k >= 1 && k < TZ / BLOCK_Y + 1) {
+            sP1[zi - 4][lx + 4] = zok[k] ? l2m * exv : T(0);
+            sP2[zi - 4][lx + 4] = zok[k] ? lam * ezv : T(0);
+            sP3x[zi - 4][lx + 4] = p3;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < KX; ++k) {
+        const int e = tid + 256 * k;
+        {
+            // lanes past the halo store into the spare row TZ
+            const bool valid = (8 * TZ) % 256 == 0 || e < 8 * TZ;
+            const int zi = valid ? e >> 3 : TZ, h = e & 7, xi = h < 4 ? h : h + TILE_X;
+            const T l2m = xl[k], mu = xm[k];
+            sP1[zi][xi] = xok[k] ? l2m * xx[k] : T(0);
+            sP2[zi][xi] = xok[k] ? (l2m - T(2) * mu) * xz[k] : T(0);
+            sP3x[zi][xi] = xok[k] ? T(2) * mu * xs[k] : T(0);
+        }
     }
     __syncthreads();
     auto body = [&](auto tagx, auto tagz) {
@@ -419,7 +471,7 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
                 }
             }
             const T b = b0[rr];
-            U[q] = u0[rr] + g.dtdx * b * D1 + g.dtdx * b * D2 + g.dtdz * b * D3;
+            stg(U, q, u0[rr] + g.dtdx * b * D1 + g.dtdx * b * D2 + g.dtdz * b * D3);
             T v = v0[rr] + g.dtdx * b * D4 + g.dtdz * b * D5 + g.dtdz * b * D6;
             T sv = 0;
             if (sh.type == 0 && r == sh.sz && c == sh.sx) {
@@ -427,7 +479,7 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
                 v += g.fsrc * b * w;
                 sv = g.fsrc * w;
             }
-            V[q] = v;
+            stg(V, q, v);
             if (a.hist2_mode == 1) {
                 const int rp = r - g.z0, cp = c - g.pad;
                 if (CZ(rp >= 0 && rp < g.nzd) && CX(cp >= 0 && cp < g.nxd)) {
@@ -463,7 +515,7 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
         return;
     }
     constexpr int RPT = TZ / BLOCK_Y;
-    __shared__ T sU[TZ + 8][TILE_X + 8], sV[TZ + 8][TILE_X + 8];
+    __shared__ T sU[TZ + 9][TILE_X + 8], sV[TZ + 9][TILE_X + 8];
     constexpr int NK = FDC<FD>::NK;
     const int ld = g.ld, nz = g.nz, nx = g.nx;
     const int tz = bx / a.tiles_x, tx = bx - tz * a.tiles_x;
@@ -475,44 +527,57 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
     T *__restrict__ ex = st + 2 * g.fstride;
     T *__restrict__ ez = st + 3 * g.fstride;
     T *__restrict__ es = st + 4 * g.fstride;
-    {
-        const int c = c0 + lx;
-#pragma unroll
-        for (int k = 0; k < (TZ + 8) / BLOCK_Y; ++k) {
-            const int zi = ty + BLOCK_Y * k, r = r0 - 4 + zi;
-            T u = 0, v = 0;
-            if (r >= 0 && r < nz && c < nx) {
-                u = U[r * ld + c];
-                v = V[r * ld + c];
-            }
-            sU[zi][lx + 4] = u;
-            sV[zi][lx + 4] = v;
-        }
-#pragma unroll
-        for (int k = 0; k < (8 * (TZ + 8) + 255) / 256; ++k) {
-            const int e = tid + 256 * k;
-            if ((8 * (TZ + 8)) % 256 == 0 || e < 8 * (TZ + 8)) {
-                const int zi = e >> 3, h = e & 7, xi = h < 4 ? h : h + TILE_X;
-                const int r = r0 - 4 + zi, cc = c0 - 4 + xi;
-                T u = 0, v = 0;
-                if (r >= 0 && r < nz && cc >= 0 && cc < nx) {
-                    u = U[r * ld + cc];
-                    v = V[r * ld + cc];
-                }
-                sU[zi][xi] = u;
-                sV[zi][xi] = v;
-            }
-        }
-    }
+    // velocities of the tile + 4-cell halo; all loads issued first, cells
+    // outside the grid load cell 0 and are zeroed by a select
+    constexpr int KZ = (TZ + 8) / BLOCK_Y, KX = (8 * (TZ + 8) + 255) / 256;
     const int c = c0 + lx;
+    T zu[KZ], zv[KZ];
+    bool zok[KZ];
+#pragma unroll
+    for (int k = 0; k < KZ; ++k) {
+        const int r = r0 - 4 + ty + BLOCK_Y * k;
+        zok[k] = r >= 0 && r < nz && c < nx;
+        const int p = zok[k] ? r * ld + c : 0;
+        zu[k] = ldg(U, p);
+        zv[k] = ldg(V, p);
+    }
+    T xu[KX], xv[KX];
+    bool xok[KX];
+#pragma unroll
+    for (int k = 0; k < KX; ++k) {
+        const int e = tid + 256 * k;
+        const int zi = e >> 3, h = e & 7, xi = h < 4 ? h : h + TILE_X;
+        const int r = r0 - 4 + zi, cc = c0 - 4 + xi;
+        xok[k] = ((8 * (TZ + 8)) % 256 == 0 || e < 8 * (TZ + 8)) && r >= 0 && r < nz && cc >= 0 && cc < nx;
+        const int p = xok[k] ? r * ld + cc : 0;
+        xu[k] = ldg(U, p);
+        xv[k] = ldg(V, p);
+    }
     T ex0[RPT], ez0[RPT], es0[RPT];
 #pragma unroll
     for (int rr = 0; rr < RPT; ++rr) {
         const int r = r0 + ty + BLOCK_Y * rr;
         const int q = (r < nz && c < nx) ? r * ld + c : 0;
-        ex0[rr] = ex[q];
-        ez0[rr] = ez[q];
-        es0[rr] = es[q];
+        ex0[rr] = ldg((const T *)ex, q);
+        ez0[rr] = ldg((const T *)ez, q);
+        es0[rr] = ldg((const T *)es, q);
+    }
+#pragma unroll
+    for (int k = 0; k < KZ; ++k) {
+        const int zi = ty + BLOCK_Y * k;
+        sU[zi][lx + 4] = zok[k] ? zu[k] : T(0);
+        sV[zi][lx + 4] = zok[k] ? zv[k] : T(0);
+    }
+#pragma unroll
+    for (int k = 0; k < KX; ++k) {
+        const int e = tid + 256 * k;
+        {
+            // lanes past the halo store into the spare row TZ + 8
+            const bool valid = (8 * (TZ + 8)) % 256 == 0 || e < 8 * (TZ + 8);
+            const int zi = valid ? e >> 3 : TZ + 8, h = e & 7, xi = h < 4 ? h : h + TILE_X;
+            sU[zi][xi] = xok[k] ? xu[k] : T(0);
+            sV[zi][xi] = xok[k] ? xv[k] : T(0);
+        }
     }
     __syncthreads();
     auto body = [&](auto tagx, auto tagz) {
@@ -558,8 +623,8 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
                 exn += stm;
                 ezn += stm;
             }
-            ex[q] = exn;
-            ez[q] = ezn;
+            stg(ex, q, exn);
+            stg(ez, q, ezn);
             // exz on [0, nz-2] x [0, nx-2]
             T esn = es0[rr];
             if (CZ(r <= nz - 2) && CX(c <= nx - 2)) {
@@ -573,15 +638,16 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
                 if (!IZ && iz >= 0) dzu = pml_fwd(dzu, pz + 4 * a.psiz_n, a.profz, r, nz);
                 if (!IX && ix >= 0) dxv = pml_fwd(dxv, px + 4 * a.psix_n, a.profx, c, nx);
                 esn = esn + T(0.5) * fma(g.dtdz, dzu, g.dtdx * dxv);
-                es[q] = esn;
+                stg(es, q, esn);
             }
             if (!IZ && a.dxu0 && r == 0) a.dxu0[((i64)s * a.nt_max + it) * nx + c] = dxu;
             const int rp = r - g.z0, cp = c - g.pad;
             if (a.hist && CZ(rp >= 0 && rp < g.nzd) && CX(cp >= 0 && cp < g.nxd)) {
-                const i64 o = ((i64)s * a.nt_max + it) * 3 * g.np + (i64)rp * g.nxd + cp;
-                a.hist[o] = exn;
-                a.hist[o + g.np] = ezn;
-                a.hist[o + 2 * g.np] = esn;
+                T *hb = a.hist + ((i64)s * a.nt_max + it) * 3 * g.np;
+                const int o = rp * g.nxd + cp, npi = (int)g.np;
+                stg(hb, o, exn);
+                stg(hb, o + npi, ezn);
+                stg(hb, o + 2 * npi, esn);
                 if (a.hist2_mode == 2) {
                     T *h = a.hist2 + ((i64)s * a.nt_max + it) * 2 * g.np + (i64)rp * g.nxd + cp;
                     h[0] = sU[zc][xc];
@@ -725,7 +791,7 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
     if (it >= sh.nt) return;
     constexpr int RPT = TZ / BLOCK_Y;
     __shared__ T szU[TZ + 8][TILE_X], szV[TZ + 8][TILE_X];
-    __shared__ T sxV[TZ][TILE_X + 8], sxU[TZ][TILE_X + 8];
+    __shared__ T sxV[TZ + 1][TILE_X + 8], sxU[TZ + 1][TILE_X + 8];
     constexpr int NK = FDC<FD>::NK;
     const int nz = g.nz, nx = g.nx, ld = g.ld;
     const int tz = bx / a.tiles_x, tx = bx - tz * a.tiles_x;
@@ -743,46 +809,106 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
 #pragma unroll
     for (int rr = 0; rr < RPT; ++rr) {
         const int r = r0 + ty + BLOCK_Y * rr;
-        const bool ok = r < nz && c < nx;
-        ua0[rr] = ok ? Ua[r * ld + c] : T(0);
-        va0[rr] = ok ? Va[r * ld + c] : T(0);
+        const int q = (r < nz && c < nx) ? r * ld + c : 0;
+        ua0[rr] = ldg((const T *)Ua, q);
+        va0[rr] = ldg((const T *)Va, q);
     }
-    // derivative adjoints after the transposed C-PML, tile + halo
+    // derivative adjoints after the transposed C-PML, tile + halo.  All
+    // loads of the stage are issued before the first use; cells outside the
+    // grid load cell 0 and are zeroed by a select.
+    constexpr int KZ = (TZ + 8) / BLOCK_Y, KX = (8 * TZ + 255) / 256;
+    const T *esa = S.esa, *exa = S.exa, *eza = S.eza;
+    T zs[KZ], zx[KZ], zz[KZ];
+    bool zok[KZ];
+#pragma unroll
+    for (int k = 0; k < KZ; ++k) {
+        const int r = r0 - 4 + ty + BLOCK_Y * k;
+        zok[k] = r >= 0 && r < nz && c < nx;
+        const int p = zok[k] ? r * ld + c : 0;
+        zs[k] = ldg(esa, p);
+        zx[k] = ldg(exa, p);
+        zz[k] = ldg(eza, p);
+    }
+    T hs[KX], hx[KX], hz[KX];
+    bool hok[KX];
+#pragma unroll
+    for (int k = 0; k < KX; ++k) {
+        const int e = tid + 256 * k;
+        const int zi = e >> 3, h = e & 7, xi = h < 4 ? h : h + TILE_X;
+        const int r = r0 + zi, cc = c0 - 4 + xi;
+        hok[k] = ((8 * TZ) % 256 == 0 || e < 8 * TZ) && r < nz && cc >= 0 && cc < nx;
+        const int p = hok[k] ? r * ld + cc : 0;
+        hs[k] = ldg(esa, p);
+        hx[k] = ldg(exa, p);
+        hz[k] = ldg(eza, p);
+    }
     if (inx && inz) {
         // no PML, no masks: the adjoints are the scaled strain adjoints
 #pragma unroll
-        for (int k = 0; k < (TZ + 8) / BLOCK_Y; ++k) {
+        for (int k = 0; k < KZ; ++k) {
             const int zi = ty + BLOCK_Y * k;
-            const int p = (r0 - 4 + zi) * ld + c;
-            const T e_s = S.esa[p];
-            szU[zi][lx] = g.hdtdz * e_s;
-            szV[zi][lx] = g.dtdz * S.eza[p];
+            szU[zi][lx] = g.hdtdz * zs[k];
+            szV[zi][lx] = g.dtdz * zz[k];
             if (k >= 1 && k < TZ / BLOCK_Y + 1) {
-                sxV[zi - 4][lx + 4] = g.hdtdx * e_s;
-                sxU[zi - 4][lx + 4] = g.dtdx * S.exa[p];
+                sxV[zi - 4][lx + 4] = g.hdtdx * zs[k];
+                sxU[zi - 4][lx + 4] = g.dtdx * zx[k];
             }
         }
 #pragma unroll
-        for (int k = 0; k < (8 * TZ + 255) / 256; ++k) {
+        for (int k = 0; k < KX; ++k) {
             const int e = tid + 256 * k;
-            if ((8 * TZ) % 256 == 0 || e < 8 * TZ) {
-                const int zi = e >> 3, h = e & 7, xi = h < 4 ? h : h + TILE_X;
-                const int p = (r0 + zi) * ld + c0 - 4 + xi;
-                sxV[zi][xi] = g.hdtdx * S.esa[p];
-                sxU[zi][xi] = g.dtdx * S.exa[p];
+            {
+                // lanes past the halo store into the spare row TZ
+                const bool valid = (8 * TZ) % 256 == 0 || e < 8 * TZ;
+                const int zi = valid ? e >> 3 : TZ, h = e & 7, xi = h < 4 ? h : h + TILE_X;
+                sxV[zi][xi] = g.hdtdx * hs[k];
+                sxU[zi][xi] = g.dtdx * hx[k];
             }
         }
     } else {
+        // z-slab: Dz_U_a, Dz_V_a everywhere, Dx_V_a, Dx_U_a on the tile rows
+        PmlIn<T> pz[KZ], px[KZ];
+        int izs[KZ], ixs[KZ];
 #pragma unroll
-        for (int k = 0; k < (TZ + 8) / BLOCK_Y; ++k) {
+        for (int k = 0; k < KZ; ++k) {
+            const int r = r0 - 4 + ty + BLOCK_Y * k;
+            izs[k] = zok[k] ? zstrip(g, r, c) : -1;
+            ixs[k] = zok[k] ? xstrip(g, r, c) : -1;
+            pz[k].load(S.pzo, S.pzo + S.pzn, izs[k], a.profz, r, nz);
+            if (k >= 1 && k < TZ / BLOCK_Y + 1) px[k].load(S.pxo, S.pxo + S.pxn, ixs[k], a.profx, c, nx);
+        }
+        PmlIn<T> hp[KX];
+        int ihs[KX];
+#pragma unroll
+        for (int k = 0; k < KX; ++k) {
+            const int e = tid + 256 * k;
+            const int zi = e >> 3, h = e & 7, xi = h < 4 ? h : h + TILE_X;
+            ihs[k] = hok[k] ? xstrip(g, r0 + zi, c0 - 4 + xi) : -1;
+            hp[k].load(S.pxo, S.pxo + S.pxn, ihs[k], a.profx, c0 - 4 + xi, nx);
+        }
+        // Dx_U_a input: dt/dx exx_a (columns >= 1) plus, on the free-surface
+        // row, the image term L_ratio dz/dx Dz_V_a (row 0 has no z-strip)
+        auto gxu = [&](int r, int cc, T exv, T ezv) {
+            T G = 0;
+            if (g.fs && r == 0 && cc <= nx - 2) G = a.Lr0[cc] * g.dzdx * (g.dtdz * ezv);
+            if (cc >= 1) G = G + g.dtdx * exv;
+            return G;
+        };
+#pragma unroll
+        for (int k = 0; k < KZ; ++k) {
             const int zi = ty + BLOCK_Y * k, r = r0 - 4 + zi;
             T u = 0, v = 0, xv = 0, xu = 0;
-            if (r >= 0 && r < nz && c < nx) {
-                u = S.DzUa(r, c);
-                v = S.DzVa(r, c);
+            if (zok[k]) {
+                const bool in2 = r <= nz - 2 && c <= nx - 2;
+                const T gzu = in2 ? g.hdtdz * zs[k] : T(0);
+                const T gzv = g.dtdz * zz[k];
+                u = izs[k] >= 0 ? pmlT_pre(gzu, pz[k].m0, pz[k].q, pz[k].a) : gzu;
+                v = izs[k] >= 0 ? pmlT_pre(gzv, pz[k].m1, pz[k].q, pz[k].a) : gzv;
                 if (k >= 1 && k < TZ / BLOCK_Y + 1) {
-                    xv = S.DxVa(r, c);
-                    xu = S.DxUa(r, c);
+                    const T gxv = in2 ? g.hdtdx * zs[k] : T(0);
+                    const T gxu_ = gxu(r, c, zx[k], zz[k]);
+                    xv = ixs[k] >= 0 ? pmlT_pre(gxv, px[k].m0, px[k].q, px[k].a) : gxv;
+                    xu = ixs[k] >= 0 ? pmlT_pre(gxu_, px[k].m1, px[k].q, px[k].a) : gxu_;
                 }
             }
             szU[zi][lx] = u;
@@ -793,15 +919,19 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
             }
         }
 #pragma unroll
-        for (int k = 0; k < (8 * TZ + 255) / 256; ++k) {
+        for (int k = 0; k < KX; ++k) {
             const int e = tid + 256 * k;
-            if ((8 * TZ) % 256 == 0 || e < 8 * TZ) {
-                const int zi = e >> 3, h = e & 7, xi = h < 4 ? h : h + TILE_X;
+            {
+                // lanes past the halo store into the spare row TZ
+                const bool valid = (8 * TZ) % 256 == 0 || e < 8 * TZ;
+                const int zi = valid ? e >> 3 : TZ, h = e & 7, xi = h < 4 ? h : h + TILE_X;
                 const int r = r0 + zi, cc = c0 - 4 + xi;
                 T xv = 0, xu = 0;
-                if (r < nz && cc >= 0 && cc < nx) {
-                    xv = S.DxVa(r, cc);
-                    xu = S.DxUa(r, cc);
+                if (hok[k]) {
+                    const T gxv = (r <= nz - 2 && cc <= nx - 2) ? g.hdtdx * hs[k] : T(0);
+                    const T gxu_ = gxu(r, cc, hx[k], hz[k]);
+                    xv = ihs[k] >= 0 ? pmlT_pre(gxv, hp[k].m0, hp[k].q, hp[k].a) : gxv;
+                    xu = ihs[k] >= 0 ? pmlT_pre(gxu_, hp[k].m1, hp[k].q, hp[k].a) : gxu_;
                 }
                 sxV[zi][xi] = xv;
                 sxU[zi][xi] = xu;
@@ -850,8 +980,8 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
                 if (CX(k == 1 ? (i1 >= 1 && i1 <= nx - 1) : (i1 >= k && i1 <= nx - k))) ua += ck * sxU[lz][xc - k + 1];
                 if (CX(k == 1 ? (i2 >= 1 && i2 <= nx - 1) : (i2 >= k && i2 <= nx - k))) ua += -ck * sxU[lz][xc + k];
             }
-            Ua[q] = ua;
-            Va[q] = va;
+            stg(Ua, q, ua);
+            stg(Va, q, va);
             // C-PML memories of this cell
             if (!IZ) {
                 const int iz = zstrip(g, r, c);
@@ -952,7 +1082,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
     const ShotDev sh = a.shots[s];
     if (it >= sh.nt) return;
     constexpr int RPT = TZ / BLOCK_Y;
-    __shared__ T sxBU[TZ][TILE_X + 8], sxMes[TZ][TILE_X + 8];
+    __shared__ T sxBU[TZ + 1][TILE_X + 8], sxMes[TZ + 1][TILE_X + 8];
     __shared__ T szBV[TZ + 8][TILE_X], szMes[TZ + 8][TILE_X];
     constexpr int NK = FDC<FD>::NK;
     const int nz = g.nz, nx = g.nx, ld = g.ld;
@@ -967,13 +1097,66 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
                      a.psix_len, a.psiz_len, a.profx, a.profz};
     const bool inx = interior_x(g, c0), inz = interior_z(g, r0, TZ);
     const int c = c0 + lx;
+    // B, U_a, V_a of the tile + halo, then the owned values and the history
+    // frame: all loads of the stage are issued before the first use; cells
+    // outside the grid load cell 0 and are zeroed by a select
+    constexpr int KZ = (TZ + 8) / BLOCK_Y, KX = (8 * TZ + 255) / 256;
+    T zb[KZ], zu[KZ], zv[KZ];
+    bool zok[KZ];
+#pragma unroll
+    for (int k = 0; k < KZ; ++k) {
+        const int r = r0 - 4 + ty + BLOCK_Y * k;
+        zok[k] = r >= 0 && r < nz && c < nx;
+        const int p = zok[k] ? r * ld + c : 0;
+        zb[k] = ldg(Bp, p);
+        zu[k] = ldg(Ua, p);
+        zv[k] = ldg(Va, p);
+    }
+    T hb[KX], hu[KX], hv[KX];
+    bool hok[KX];
+#pragma unroll
+    for (int k = 0; k < KX; ++k) {
+        const int e = tid + 256 * k;
+        const int zi = e >> 3, h = e & 7, xi = h < 4 ? h : h + TILE_X;
+        const int r = r0 + zi, cc = c0 - 4 + xi;
+        hok[k] = ((8 * TZ) % 256 == 0 || e < 8 * TZ) && r < nz && cc >= 0 && cc < nx;
+        const int p = hok[k] ? r * ld + cc : 0;
+        hb[k] = ldg(Bp, p);
+        hu[k] = ldg(Ua, p);
+        hv[k] = ldg(Va, p);
+    }
+    // owned values and the history frame, loaded ahead of the barrier
+    T l2m0[RPT], mu0[RPT], exa0v[RPT], eza0v[RPT], esa0v[RPT], h0[RPT], h1[RPT], h2v[RPT];
+    const i64 hs = (i64)s * a.nt_max;
+    const int hframe = a.imaging == 2 ? it - 1 : it;
+#pragma unroll
+    for (int rr = 0; rr < RPT; ++rr) {
+        const int r = r0 + ty + BLOCK_Y * rr;
+        const int q = (r < nz && c < nx) ? r * ld + c : 0;
+        l2m0[rr] = ldg(a.L2M + g.origin, q);
+        mu0[rr] = ldg(a.M + g.origin, q);
+        exa0v[rr] = ldg((const T *)exa, q);
+        eza0v[rr] = ldg((const T *)eza, q);
+        esa0v[rr] = ldg((const T *)esa, q);
+        // history frame: read wherever the imaging below uses it (physical
+        // cells; frame -1 is never used); other lanes read entry 0
+        h0[rr] = h1[rr] = h2v[rr] = T(0);
+        if (a.imaging) {
+            const int rp = r - g.z0, cp = c - g.pad;
+            const bool hok = hframe >= 0 && rp >= 0 && rp < g.nzd && cp >= 0 && cp < g.nxd;
+            const T *hfr = a.hist + (hs + (hframe >= 0 ? hframe : 0)) * 3 * g.np;
+            const int o = hok ? rp * g.nxd + cp : 0, npi = (int)g.np;
+            h0[rr] = ldg(hfr, o);
+            h1[rr] = ldg(hfr, o + npi);
+            h2v[rr] = ldg(hfr, o + 2 * npi);
+        }
+    }
     if (inx && inz) {
         // no PML, no masks: Dx_(BU)_a = dt/dx B U_a, Dz_M_es_a = dt/dz B U_a, ...
 #pragma unroll
-        for (int k = 0; k < (TZ + 8) / BLOCK_Y; ++k) {
+        for (int k = 0; k < KZ; ++k) {
             const int zi = ty + BLOCK_Y * k;
-            const int p = (r0 - 4 + zi) * ld + c;
-            const T b = Bp[p], u = Ua[p], v = Va[p];
+            const T b = zb[k], u = zu[k], v = zv[k];
             szBV[zi][lx] = g.dtdz * b * v;
             szMes[zi][lx] = g.dtdz * b * u;
             if (k >= 1 && k < TZ / BLOCK_Y + 1) {
@@ -982,27 +1165,55 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
             }
         }
 #pragma unroll
-        for (int k = 0; k < (8 * TZ + 255) / 256; ++k) {
+        for (int k = 0; k < KX; ++k) {
             const int e = tid + 256 * k;
-            if ((8 * TZ) % 256 == 0 || e < 8 * TZ) {
-                const int zi = e >> 3, h = e & 7, xi = h < 4 ? h : h + TILE_X;
-                const int p = (r0 + zi) * ld + c0 - 4 + xi;
-                const T b = Bp[p];
-                sxBU[zi][xi] = g.dtdx * b * Ua[p];
-                sxMes[zi][xi] = g.dtdx * b * Va[p];
+            {
+                // lanes past the halo store into the spare row TZ
+                const bool valid = (8 * TZ) % 256 == 0 || e < 8 * TZ;
+                const int zi = valid ? e >> 3 : TZ, h = e & 7, xi = h < 4 ? h : h + TILE_X;
+                const T b = hb[k];
+                sxBU[zi][xi] = g.dtdx * b * hu[k];
+                sxMes[zi][xi] = g.dtdx * b * hv[k];
             }
         }
     } else {
+        // memories: x[2] = Dx_M_es, x[3] = Dx_(B U); z[2] = Dz_(B V), z[3] = Dz_M_es
+        PmlIn<T> pz[KZ], px[KZ];
+        int izs[KZ], ixs[KZ];
 #pragma unroll
-        for (int k = 0; k < (TZ + 8) / BLOCK_Y; ++k) {
+        for (int k = 0; k < KZ; ++k) {
+            const int r = r0 - 4 + ty + BLOCK_Y * k;
+            izs[k] = zok[k] ? zstrip(g, r, c) : -1;
+            ixs[k] = zok[k] ? xstrip(g, r, c) : -1;
+            pz[k].load(S.pzo + 2 * S.pzn, S.pzo + 3 * S.pzn, izs[k], a.profz, r, nz);
+            if (k >= 1 && k < TZ / BLOCK_Y + 1)
+                px[k].load(S.pxo + 2 * S.pxn, S.pxo + 3 * S.pxn, ixs[k], a.profx, c, nx);
+        }
+        PmlIn<T> hp[KX];
+        int ihs[KX];
+#pragma unroll
+        for (int k = 0; k < KX; ++k) {
+            const int e = tid + 256 * k;
+            const int zi = e >> 3, h = e & 7, xi = h < 4 ? h : h + TILE_X;
+            ihs[k] = hok[k] ? xstrip(g, r0 + zi, c0 - 4 + xi) : -1;
+            hp[k].load(S.pxo + 2 * S.pxn, S.pxo + 3 * S.pxn, ihs[k], a.profx, c0 - 4 + xi, nx);
+        }
+#pragma unroll
+        for (int k = 0; k < KZ; ++k) {
             const int zi = ty + BLOCK_Y * k, r = r0 - 4 + zi;
             T bv = 0, mz = 0, bu = 0, mx = 0;
-            if (r >= 0 && r < nz && c < nx) {
-                bv = S.DzBVa(r, c);
-                mz = S.DzMesa(r, c);
+            if (zok[k]) {
+                const bool inR = r <= nz - 2 && c >= 1 && c <= nx - 2;
+                const T b = zb[k], u = zu[k], v = zv[k];
+                const T gzbv = inR ? g.dtdz * b * v : T(0);
+                const T gzmes = inR ? g.dtdz * b * u : T(0);
+                bv = izs[k] >= 0 ? pmlT_pre(gzbv, pz[k].m0, pz[k].q, pz[k].a) : gzbv;
+                mz = izs[k] >= 0 ? pmlT_pre(gzmes, pz[k].m1, pz[k].q, pz[k].a) : gzmes;
                 if (k >= 1 && k < TZ / BLOCK_Y + 1) {
-                    bu = S.DxBUa(r, c);
-                    mx = S.DxMesa(r, c);
+                    const T gxbu = inR ? g.dtdx * b * u : T(0);
+                    const T gxmes = inR ? g.dtdx * b * v : T(0);
+                    bu = ixs[k] >= 0 ? pmlT_pre(gxbu, px[k].m1, px[k].q, px[k].a) : gxbu;
+                    mx = ixs[k] >= 0 ? pmlT_pre(gxmes, px[k].m0, px[k].q, px[k].a) : gxmes;
                 }
             }
             szBV[zi][lx] = bv;
@@ -1013,41 +1224,25 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
             }
         }
 #pragma unroll
-        for (int k = 0; k < (8 * TZ + 255) / 256; ++k) {
+        for (int k = 0; k < KX; ++k) {
             const int e = tid + 256 * k;
-            if ((8 * TZ) % 256 == 0 || e < 8 * TZ) {
-                const int zi = e >> 3, h = e & 7, xi = h < 4 ? h : h + TILE_X;
+            {
+                // lanes past the halo store into the spare row TZ
+                const bool valid = (8 * TZ) % 256 == 0 || e < 8 * TZ;
+                const int zi = valid ? e >> 3 : TZ, h = e & 7, xi = h < 4 ? h : h + TILE_X;
                 const int r = r0 + zi, cc = c0 - 4 + xi;
                 T bu = 0, mx = 0;
-                if (r < nz && cc >= 0 && cc < nx) {
-                    bu = S.DxBUa(r, cc);
-                    mx = S.DxMesa(r, cc);
+                if (hok[k]) {
+                    const bool inR = r <= nz - 2 && cc >= 1 && cc <= nx - 2;
+                    const T b = hb[k];
+                    const T gxbu = inR ? g.dtdx * b * hu[k] : T(0);
+                    const T gxmes = inR ? g.dtdx * b * hv[k] : T(0);
+                    bu = ihs[k] >= 0 ? pmlT_pre(gxbu, hp[k].m1, hp[k].q, hp[k].a) : gxbu;
+                    mx = ihs[k] >= 0 ? pmlT_pre(gxmes, hp[k].m0, hp[k].q, hp[k].a) : gxmes;
                 }
                 sxBU[zi][xi] = bu;
                 sxMes[zi][xi] = mx;
             }
-        }
-    }
-    // owned values and the history frame, loaded ahead of the barrier
-    T l2m0[RPT], mu0[RPT], exa0v[RPT], eza0v[RPT], esa0v[RPT], h0[RPT], h1[RPT], h2v[RPT];
-    const i64 hs = (i64)s * a.nt_max;
-    const int hframe = a.imaging == 2 ? it - 1 : it;
-#pragma unroll
-    for (int rr = 0; rr < RPT; ++rr) {
-        const int r = r0 + ty + BLOCK_Y * rr;
-        const int q = (r < nz && c < nx) ? r * ld + c : 0;
-        l2m0[rr] = a.L2M[g.origin + q];
-        mu0[rr] = a.M[g.origin + q];
-        exa0v[rr] = exa[q];
-        eza0v[rr] = eza[q];
-        esa0v[rr] = esa[q];
-        const int rp = r - g.z0, cp = c - g.pad;
-        h0[rr] = h1[rr] = h2v[rr] = T(0);
-        if (a.imaging && hframe >= 0 && rp >= 0 && rp < g.nzd && cp >= 0 && cp < g.nxd) {
-            const T *h = a.hist + (hs + hframe) * 3 * g.np + (i64)rp * g.nxd + cp;
-            h0[rr] = h[0];
-            h1[rr] = h[g.np];
-            h2v[rr] = h[2 * g.np];
         }
     }
     __syncthreads();
@@ -1099,9 +1294,9 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
             const T l2m = l2m0[rr], mu = mu0[rr];
             const T lam = l2m - T(2) * mu;
             const T exa0 = exa0v[rr], eza0 = eza0v[rr];
-            exa[q] = exa0 + l2m * Ax + lam * Az;
-            eza[q] = eza0 + lam * Ax + l2m * Az;
-            esa[q] = esa0v[rr] + T(2) * mu * Mes;
+            stg(exa, q, exa0 + l2m * Ax + lam * Az);
+            stg(eza, q, eza0 + lam * Ax + l2m * Az);
+            stg(esa, q, esa0v[rr] + T(2) * mu * Mes);
 
             if (!IX) {
                 const int ix = xstrip(g, r, c);
@@ -1133,7 +1328,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
             // imaging on the physical cells
             const int rp = r - g.z0, cp = c - g.pad;
             if (a.imaging == 0 || !(CZ(rp >= 0 && rp < g.nzd) && CX(cp >= 0 && cp < g.nxd))) continue;
-            const i64 ip = (i64)rp * g.nxd + cp;
+            const int ip = rp * g.nxd + cp;
             T *acc = a.gacc + (i64)s * a.nacc * g.np + ip;
             if (a.imaging == 2) {
                 // exact discrete gradient: model dependence of step it, paired with

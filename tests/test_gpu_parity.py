@@ -222,9 +222,9 @@ def test_gpu_tile_heights_agree_bitwise():
     p.set_model(m["Vs"], m["Vp"], m["rho"])
     p.set_shots([77], [4], "moment", m["nt"], rx, rz, wav)
     tiles, table = p.autotune(3)
-    assert all(t in (8, 16, 32) for t in tiles) and set(table) == {"fwd_vel", "fwd_strain", "adj_strain", "adj_vel"}
+    assert all(t in (4, 8, 16, 32) for t in tiles) and set(table) == {"fwd_vel", "fwd_strain", "adj_strain", "adj_vel"}
     outs = []
-    for tz in ([8] * 4, [16] * 4, [32] * 4, [32, 8, 16, 8]):
+    for tz in ([8] * 4, [4] * 4, [16] * 4, [32] * 4, [32, 4, 16, 8]):
         p.set_tiles(tz)
         rec = p.forward()["rec"]
         _, radj = p.residual(rec, obs, 2, None, 0, 1e15, 1)
