@@ -126,7 +126,7 @@ __device__ __forceinline__ T pmlT_pre(T G, T psi_old, T q, T a)
 // (i < 0) read entry 0, which a select discards afterwards.
 template <typename T>
 struct PmlIn {
-    T m0, m1, q, a;
+    T m0, m1, q, a, b;
     __device__ __forceinline__ void load(const T *mem0, const T *mem1, int i, const T *prof, int j, int n)
     {
         const int ic = i >= 0 ? i : 0;
@@ -135,8 +135,18 @@ struct PmlIn {
         m1 = ldg(mem1, ic);
         q = ldg(prof + 3 * n, jc);
         a = ldg(prof + 2 * n, jc);
+        b = ldg(prof + n, jc);
     }
 };
+
+// pml_fwd with the old memory already loaded
+template <typename T>
+__device__ __forceinline__ T pml_fwd_pre(T D, T psi_old, T *psi, const T *prof, int j, int n)
+{
+    const T pn = fma(prof[n + j], psi_old, prof[2 * n + j] * D);
+    *psi = pn;
+    return D / prof[j] + pn;
+}
 
 // ---------------------------------------------------------------------------
 // model set-up (fwi/forward.py:193-238)
@@ -385,14 +395,34 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
         xz[k] = ldg(ez, p);
         xs[k] = ldg(es, p);
     }
-    T u0[RPT], v0[RPT], b0[RPT];
+    // owned values and, in tiles touching a strip, the owned cells' C-PML
+    // memories (x: Dx sxx, Dx sxz-adjacent terms 0..2; z: 0..2)
+    T u0[RPT], v0[RPT], b0[RPT], mx[RPT][3], mz[RPT][3];
+    int ixo[RPT], izo[RPT];
+    const bool inx = interior_x(g, c0), inz = interior_z(g, r0, TZ);
 #pragma unroll
     for (int rr = 0; rr < RPT; ++rr) {
         const int r = r0 + ty + BLOCK_Y * rr;
-        const int q = (r < nz && c < nx) ? r * ld + c : 0;
+        const bool ok = r < nz && c < nx;
+        const int q = ok ? r * ld + c : 0;
         u0[rr] = ldg((const T *)U, q);
         v0[rr] = ldg((const T *)V, q);
         b0[rr] = ldg(B, q);
+        ixo[rr] = izo[rr] = -1;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) mx[rr][j] = mz[rr][j] = T(0);
+        if (!inx) {
+            ixo[rr] = ok ? xstrip(g, r, c) : -1;
+            const T *pb = a.psix + s * a.psix_stride;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) mx[rr][j] = ldg(pb + j * a.psix_n, ixo[rr] >= 0 ? ixo[rr] : 0);
+        }
+        if (!inz) {
+            izo[rr] = ok ? zstrip(g, r, c) : -1;
+            const T *pb = a.psiz + s * a.psiz_stride;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) mz[rr][j] = ldg(pb + j * a.psiz_n, izo[rr] >= 0 ? izo[rr] : 0);
+        }
     }
 #pragma unroll
     for (int k = 0; k < KZ; ++k) {
@@ -453,21 +483,21 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
             }
             if (!IZ && g.fs && r == 0) D3 = T(2 * fdc<FD>(1)) * sP3z[zc][lx];   // image of sigma_xz at the surface
             if (!IX) {
-                const int ix = xstrip(g, r, c);
+                const int ix = ixo[rr];
                 if (ix >= 0) {
                     T *px = a.psix + s * a.psix_stride + ix;
-                    D1 = pml_fwd(D1, px + 0 * a.psix_n, a.profx, c, nx);
-                    D2 = pml_fwd(D2, px + 1 * a.psix_n, a.profx, c, nx);
-                    D4 = pml_fwd(D4, px + 2 * a.psix_n, a.profx, c, nx);
+                    D1 = pml_fwd_pre(D1, mx[rr][0], px + 0 * a.psix_n, a.profx, c, nx);
+                    D2 = pml_fwd_pre(D2, mx[rr][1], px + 1 * a.psix_n, a.profx, c, nx);
+                    D4 = pml_fwd_pre(D4, mx[rr][2], px + 2 * a.psix_n, a.profx, c, nx);
                 }
             }
             if (!IZ) {
-                const int iz = zstrip(g, r, c);
+                const int iz = izo[rr];
                 if (iz >= 0) {
                     T *pz = a.psiz + s * a.psiz_stride + iz;
-                    D3 = pml_fwd(D3, pz + 0 * a.psiz_n, a.profz, r, nz);
-                    D5 = pml_fwd(D5, pz + 1 * a.psiz_n, a.profz, r, nz);
-                    D6 = pml_fwd(D6, pz + 2 * a.psiz_n, a.profz, r, nz);
+                    D3 = pml_fwd_pre(D3, mz[rr][0], pz + 0 * a.psiz_n, a.profz, r, nz);
+                    D5 = pml_fwd_pre(D5, mz[rr][1], pz + 1 * a.psiz_n, a.profz, r, nz);
+                    D6 = pml_fwd_pre(D6, mz[rr][2], pz + 2 * a.psiz_n, a.profz, r, nz);
                 }
             }
             const T b = b0[rr];
@@ -490,7 +520,7 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
             }
         }
     };
-    STFWI_DISPATCH(interior_x(g, c0), interior_z(g, r0, TZ), body);
+    STFWI_DISPATCH(inx, inz, body);
 }
 
 // strain half-step (fwi/forward.py:369-444) + vx/vz receivers, history,
@@ -553,14 +583,35 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
         xu[k] = ldg(U, p);
         xv[k] = ldg(V, p);
     }
-    T ex0[RPT], ez0[RPT], es0[RPT];
+    // owned strains and, in tiles touching a strip, the owned cells' C-PML
+    // memories 3 (exx / ezz derivatives) and 4 (exz derivatives)
+    T ex0[RPT], ez0[RPT], es0[RPT], mx3[RPT], mx4[RPT], mz3[RPT], mz4[RPT];
+    int ixo[RPT], izo[RPT];
+    const bool inx = interior_x(g, c0), inz = interior_z(g, r0, TZ);
 #pragma unroll
     for (int rr = 0; rr < RPT; ++rr) {
         const int r = r0 + ty + BLOCK_Y * rr;
-        const int q = (r < nz && c < nx) ? r * ld + c : 0;
+        const bool ok = r < nz && c < nx;
+        const int q = ok ? r * ld + c : 0;
         ex0[rr] = ldg((const T *)ex, q);
         ez0[rr] = ldg((const T *)ez, q);
         es0[rr] = ldg((const T *)es, q);
+        ixo[rr] = izo[rr] = -1;
+        mx3[rr] = mx4[rr] = mz3[rr] = mz4[rr] = T(0);
+        if (!inx) {
+            ixo[rr] = ok ? xstrip(g, r, c) : -1;
+            const T *pb = a.psix + s * a.psix_stride;
+            const int i = ixo[rr] >= 0 ? ixo[rr] : 0;
+            mx3[rr] = ldg(pb + 3 * a.psix_n, i);
+            mx4[rr] = ldg(pb + 4 * a.psix_n, i);
+        }
+        if (!inz) {
+            izo[rr] = ok ? zstrip(g, r, c) : -1;
+            const T *pb = a.psiz + s * a.psiz_stride;
+            const int i = izo[rr] >= 0 ? izo[rr] : 0;
+            mz3[rr] = ldg(pb + 3 * a.psiz_n, i);
+            mz4[rr] = ldg(pb + 4 * a.psiz_n, i);
+        }
     }
 #pragma unroll
     for (int k = 0; k < KZ; ++k) {
@@ -596,12 +647,10 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
                     if (CX(k == 1 || (c >= k && c <= nx - k)))
                         dxu += (T)fdc<FD>(k) * (sU[zc][xc + k - 1] - sU[zc][xc - k]);
             }
-            int ix = -1, iz = -1;
-            if (!IX) ix = xstrip(g, r, c);
-            if (!IZ) iz = zstrip(g, r, c);
+            const int ix = IX ? -1 : ixo[rr], iz = IZ ? -1 : izo[rr];
             T *px = a.psix + s * a.psix_stride + ix;
             T *pz = a.psiz + s * a.psiz_stride + iz;
-            if (!IX && ix >= 0) dxu = pml_fwd(dxu, px + 3 * a.psix_n, a.profx, c, nx);
+            if (!IX && ix >= 0) dxu = pml_fwd_pre(dxu, mx3[rr], px + 3 * a.psix_n, a.profx, c, nx);
             T exn = CX(c >= 1) ? ex0[rr] + g.dtdx * dxu : ex0[rr];
             // ezz: D-z V on columns [0, nx-2]; free-surface row from the C-PML'd D-x U
             T dzv = 0;
@@ -615,7 +664,7 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
                             dzv += (T)fdc<FD>(k) * (sV[zc + k - 1][xc] - sV[zc - k][xc]);
                 }
             }
-            if (!IZ && iz >= 0) dzv = pml_fwd(dzv, pz + 3 * a.psiz_n, a.profz, r, nz);
+            if (!IZ && iz >= 0) dzv = pml_fwd_pre(dzv, mz3[rr], pz + 3 * a.psiz_n, a.profz, r, nz);
             T ezn = ez0[rr] + g.dtdz * dzv;
             if (sh.type == 1 && r == sh.sz && c == sh.sx) {
                 const T lm = (a.L2M[g.origin + q] - T(2) * a.M[g.origin + q]) + a.M[g.origin + q];
@@ -635,8 +684,8 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
                     if (CX(c >= 1) && CZ(r >= k - 1 && r <= nz - 1 - k)) dzu += ck * (sU[zc + k][xc] - sU[zc - k + 1][xc]);
                     if (CX(c >= k - 1 && c <= nx - 1 - k)) dxv += ck * (sV[zc][xc + k] - sV[zc][xc - k + 1]);
                 }
-                if (!IZ && iz >= 0) dzu = pml_fwd(dzu, pz + 4 * a.psiz_n, a.profz, r, nz);
-                if (!IX && ix >= 0) dxv = pml_fwd(dxv, px + 4 * a.psix_n, a.profx, c, nx);
+                if (!IZ && iz >= 0) dzu = pml_fwd_pre(dzu, mz4[rr], pz + 4 * a.psiz_n, a.profz, r, nz);
+                if (!IX && ix >= 0) dxv = pml_fwd_pre(dxv, mx4[rr], px + 4 * a.psix_n, a.profx, c, nx);
                 esn = esn + T(0.5) * fma(g.dtdz, dzu, g.dtdx * dxv);
                 stg(es, q, esn);
             }
@@ -670,7 +719,7 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
             }
         }
     };
-    STFWI_DISPATCH(interior_x(g, c0), interior_z(g, r0, TZ), body);
+    STFWI_DISPATCH(inx, inz, body);
 }
 
 
@@ -842,6 +891,15 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
         hx[k] = ldg(exa, p);
         hz[k] = ldg(eza, p);
     }
+    // C-PML inputs of the staged cells and the owned cells' G values, kept
+    // for the memory update after the barrier (tiles touching a strip only)
+    PmlIn<T> pz[KZ], px[KZ];
+    int izs[KZ], ixs[KZ];
+    T gzu_o[RPT], gzv_o[RPT], gxv_o[RPT], gxu_o[RPT];
+#pragma unroll
+    for (int k = 0; k < KZ; ++k) izs[k] = ixs[k] = -1;
+#pragma unroll
+    for (int rr = 0; rr < RPT; ++rr) gzu_o[rr] = gzv_o[rr] = gxv_o[rr] = gxu_o[rr] = T(0);
     if (inx && inz) {
         // no PML, no masks: the adjoints are the scaled strain adjoints
 #pragma unroll
@@ -867,8 +925,6 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
         }
     } else {
         // z-slab: Dz_U_a, Dz_V_a everywhere, Dx_V_a, Dx_U_a on the tile rows
-        PmlIn<T> pz[KZ], px[KZ];
-        int izs[KZ], ixs[KZ];
 #pragma unroll
         for (int k = 0; k < KZ; ++k) {
             const int r = r0 - 4 + ty + BLOCK_Y * k;
@@ -909,6 +965,10 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
                     const T gxu_ = gxu(r, c, zx[k], zz[k]);
                     xv = ixs[k] >= 0 ? pmlT_pre(gxv, px[k].m0, px[k].q, px[k].a) : gxv;
                     xu = ixs[k] >= 0 ? pmlT_pre(gxu_, px[k].m1, px[k].q, px[k].a) : gxu_;
+                    gzu_o[k - 1] = gzu;
+                    gzv_o[k - 1] = gzv;
+                    gxv_o[k - 1] = gxv;
+                    gxu_o[k - 1] = gxu_;
                 }
             }
             szU[zi][lx] = u;
@@ -982,21 +1042,24 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
             }
             stg(Ua, q, ua);
             stg(Va, q, va);
-            // C-PML memories of this cell
+            // C-PML memories of this cell: psi' = b (G + psi), from the values
+            // staged for row rr + 1 of the z-slab
             if (!IZ) {
-                const int iz = zstrip(g, r, c);
+                const int iz = izs[rr + 1];
                 if (iz >= 0) {
-                    T *pz = a.psiz_n + s * a.psiz_stride + iz;
-                    pz[0] = pmlT_psi(S.GzU(r, c), S.pzo[iz], a.profz, r, nz);
-                    pz[a.psiz_len] = pmlT_psi(S.GzV(r, c), S.pzo[a.psiz_len + iz], a.profz, r, nz);
+                    T *pzn = a.psiz_n + s * a.psiz_stride + iz;
+                    const PmlIn<T> &m = pz[rr + 1];
+                    pzn[0] = m.b * (gzu_o[rr] + m.m0);
+                    pzn[a.psiz_len] = m.b * (gzv_o[rr] + m.m1);
                 }
             }
             if (!IX) {
-                const int ix = xstrip(g, r, c);
+                const int ix = ixs[rr + 1];
                 if (ix >= 0) {
-                    T *px = a.psix_n + s * a.psix_stride + ix;
-                    px[0] = pmlT_psi(S.GxV(r, c), S.pxo[ix], a.profx, c, nx);
-                    px[a.psix_len] = pmlT_psi(S.GxU(r, c), S.pxo[a.psix_len + ix], a.profx, c, nx);
+                    T *pxn = a.psix_n + s * a.psix_stride + ix;
+                    const PmlIn<T> &m = px[rr + 1];
+                    pxn[0] = m.b * (gxv_o[rr] + m.m0);
+                    pxn[a.psix_len] = m.b * (gxu_o[rr] + m.m1);
                 }
             }
             if (r == sh.sz && c == sh.sx && a.src_adj) {
@@ -1126,7 +1189,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
         hv[k] = ldg(Va, p);
     }
     // owned values and the history frame, loaded ahead of the barrier
-    T l2m0[RPT], mu0[RPT], exa0v[RPT], eza0v[RPT], esa0v[RPT], h0[RPT], h1[RPT], h2v[RPT];
+    T l2m0[RPT], mu0[RPT], exa0v[RPT], eza0v[RPT], esa0v[RPT], h0[RPT], h1[RPT], h2v[RPT], ac0[RPT], ac1[RPT];
     const i64 hs = (i64)s * a.nt_max;
     const int hframe = a.imaging == 2 ? it - 1 : it;
 #pragma unroll
@@ -1140,17 +1203,32 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
         esa0v[rr] = ldg((const T *)esa, q);
         // history frame: read wherever the imaging below uses it (physical
         // cells; frame -1 is never used); other lanes read entry 0
-        h0[rr] = h1[rr] = h2v[rr] = T(0);
+        h0[rr] = h1[rr] = h2v[rr] = ac0[rr] = ac1[rr] = T(0);
         if (a.imaging) {
             const int rp = r - g.z0, cp = c - g.pad;
-            const bool hok = hframe >= 0 && rp >= 0 && rp < g.nzd && cp >= 0 && cp < g.nxd;
+            const bool pok = rp >= 0 && rp < g.nzd && cp >= 0 && cp < g.nxd;
+            const bool hok = hframe >= 0 && pok;
             const T *hfr = a.hist + (hs + (hframe >= 0 ? hframe : 0)) * 3 * g.np;
             const int o = hok ? rp * g.nxd + cp : 0, npi = (int)g.np;
             h0[rr] = ldg(hfr, o);
             h1[rr] = ldg(hfr, o + npi);
             h2v[rr] = ldg(hfr, o + 2 * npi);
+            // first two gradient accumulators of the cell
+            const T *accb = a.gacc + (i64)s * a.nacc * g.np;
+            const int oa = pok ? rp * g.nxd + cp : 0;
+            ac0[rr] = ldg(accb, oa);
+            if (a.gparams >= 2) ac1[rr] = ldg(accb, oa + npi);
         }
     }
+    // C-PML inputs of the staged cells and the owned cells' G values, kept
+    // for the memory update after the barrier (tiles touching a strip only)
+    PmlIn<T> pz[KZ], px[KZ];
+    int izs[KZ], ixs[KZ];
+    T gzbv_o[RPT], gzmes_o[RPT], gxbu_o[RPT], gxmes_o[RPT];
+#pragma unroll
+    for (int k = 0; k < KZ; ++k) izs[k] = ixs[k] = -1;
+#pragma unroll
+    for (int rr = 0; rr < RPT; ++rr) gzbv_o[rr] = gzmes_o[rr] = gxbu_o[rr] = gxmes_o[rr] = T(0);
     if (inx && inz) {
         // no PML, no masks: Dx_(BU)_a = dt/dx B U_a, Dz_M_es_a = dt/dz B U_a, ...
 #pragma unroll
@@ -1178,8 +1256,6 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
         }
     } else {
         // memories: x[2] = Dx_M_es, x[3] = Dx_(B U); z[2] = Dz_(B V), z[3] = Dz_M_es
-        PmlIn<T> pz[KZ], px[KZ];
-        int izs[KZ], ixs[KZ];
 #pragma unroll
         for (int k = 0; k < KZ; ++k) {
             const int r = r0 - 4 + ty + BLOCK_Y * k;
@@ -1214,6 +1290,10 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
                     const T gxmes = inR ? g.dtdx * b * v : T(0);
                     bu = ixs[k] >= 0 ? pmlT_pre(gxbu, px[k].m1, px[k].q, px[k].a) : gxbu;
                     mx = ixs[k] >= 0 ? pmlT_pre(gxmes, px[k].m0, px[k].q, px[k].a) : gxmes;
+                    gzbv_o[k - 1] = gzbv;
+                    gzmes_o[k - 1] = gzmes;
+                    gxbu_o[k - 1] = gxbu;
+                    gxmes_o[k - 1] = gxmes;
                 }
             }
             szBV[zi][lx] = bv;
@@ -1298,20 +1378,24 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
             stg(eza, q, eza0 + lam * Ax + l2m * Az);
             stg(esa, q, esa0v[rr] + T(2) * mu * Mes);
 
+            // C-PML memories of this cell: psi' = b (G + psi), from the values
+            // staged for row rr + 1 of the z-slab
             if (!IX) {
-                const int ix = xstrip(g, r, c);
+                const int ix = ixs[rr + 1];
                 if (ix >= 0) {
-                    T *px = a.psix_n + s * a.psix_stride + ix;
-                    px[2 * a.psix_len] = pmlT_psi(S.GxMes(r, c), S.pxo[2 * a.psix_len + ix], a.profx, c, nx);
-                    px[3 * a.psix_len] = pmlT_psi(S.GxBU(r, c), S.pxo[3 * a.psix_len + ix], a.profx, c, nx);
+                    T *pxn = a.psix_n + s * a.psix_stride + ix;
+                    const PmlIn<T> &m = px[rr + 1];
+                    pxn[2 * a.psix_len] = m.b * (gxmes_o[rr] + m.m0);
+                    pxn[3 * a.psix_len] = m.b * (gxbu_o[rr] + m.m1);
                 }
             }
             if (!IZ) {
-                const int iz = zstrip(g, r, c);
+                const int iz = izs[rr + 1];
                 if (iz >= 0) {
-                    T *pz = a.psiz_n + s * a.psiz_stride + iz;
-                    pz[2 * a.psiz_len] = pmlT_psi(S.GzBV(r, c), S.pzo[2 * a.psiz_len + iz], a.profz, r, nz);
-                    pz[3 * a.psiz_len] = pmlT_psi(S.GzMes(r, c), S.pzo[3 * a.psiz_len + iz], a.profz, r, nz);
+                    T *pzn = a.psiz_n + s * a.psiz_stride + iz;
+                    const PmlIn<T> &m = pz[rr + 1];
+                    pzn[2 * a.psiz_len] = m.b * (gzbv_o[rr] + m.m0);
+                    pzn[3 * a.psiz_len] = m.b * (gzmes_o[rr] + m.m1);
                 }
             }
 
@@ -1356,13 +1440,13 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
                 if (a.gparams == 3) {
                     const T *h2 = a.hist2 + (hs + it) * 2 * g.np + ip;
                     gb = fma(Ua[q], h2[0], Va[q] * h2[g.np]);
-                    acc[0] += gl2m;
-                    acc[g.np] += gl;
+                    acc[0] = ac0[rr] + gl2m;
+                    acc[g.np] = ac1[rr] + gl;
                     acc[2 * g.np] += gm;
                     acc[3 * g.np] += gb;
                 } else {
-                    acc[0] += gm - T(2) * gl;
-                    if (a.gparams == 2) acc[g.np] += gl2m + gl;
+                    acc[0] = ac0[rr] + (gm - T(2) * gl);
+                    if (a.gparams == 2) acc[g.np] = ac1[rr] + (gl2m + gl);
                 }
             } else {
                 // Zhou imaging (fwi/imaging.py:125-152): forward frame it with the
@@ -1393,12 +1477,12 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
                         const T *h2 = a.hist2 + (hs + it + 1) * 2 * g.np + ip;
                         grl -= Ua[q] * h2[0] / i2dt + Va[q] * h2[g.np] / i2dt;
                     }
-                    acc[0] += glam;
-                    acc[g.np] += gmu;
+                    acc[0] = ac0[rr] + glam;
+                    acc[g.np] = ac1[rr] + gmu;
                     acc[2 * g.np] += grl;
                 } else {
-                    acc[0] += gmu - T(2) * glam;
-                    if (a.gparams == 2) acc[g.np] += glam;
+                    acc[0] = ac0[rr] + (gmu - T(2) * glam);
+                    if (a.gparams == 2) acc[g.np] = ac1[rr] + glam;
                 }
             }
         }
