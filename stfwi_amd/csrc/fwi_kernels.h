@@ -77,18 +77,66 @@ __device__ __forceinline__ int zstrip(const Geo<T> &g, int r, int c)
     return -1;
 }
 
-// Global accesses of the time-step kernels: a block-uniform base pointer
-// plus a non-negative 32-bit element index, turned into a 32-bit byte offset
-// so the compiler emits base(SGPR) + offset(VGPR) addressing instead of
-// per-lane 64-bit pointer arithmetic.  Every index passed is >= 0.
+// Global accesses of the time-step kernels.  ldg/stg: raw buffer loads and
+// stores through a resource built from a block-uniform base pointer (kept in
+// SGPRs) with a 32-bit byte offset per lane, so no per-lane 64-bit address
+// arithmetic is emitted.  The resource's range is unbounded: every index
+// passed is >= 0 and inside the allocation.  Word 3 is the CDNA (gfx9
+// family) raw-buffer configuration.
+#define STFWI_BUF_WORD3 0x00020000
+
+template <typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const T *base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(base), (short)0, -1, STFWI_BUF_WORD3);
+}
+
+__device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t r, const float *, int idx)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, idx * 4, 0, 0));
+}
+
+__device__ __forceinline__ double buf_ld(__amdgpu_buffer_rsrc_t r, const double *, int idx)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, idx * 8, 0, 0));
+}
+
+__device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, float v, int idx)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, idx * 4, 0, 0);
+}
+
+__device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, double v, int idx)
+{
+    typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, idx * 8, 0, 0);
+}
+
 template <typename T>
 __device__ __forceinline__ T ldg(const T *base, int idx)
+{
+    return buf_ld(buf_rsrc(base), base, idx);
+}
+
+template <typename T>
+__device__ __forceinline__ void stg(T *base, int idx, T v)
+{
+    buf_st(buf_rsrc(base), v, idx);
+}
+
+// ldgg/stgg: the same accesses as plain global loads and stores with a
+// 32-bit unsigned byte offset from the uniform base.  The velocity-forward
+// and strain-adjoint kernels use the buffer form, the strain-forward and
+// velocity-adjoint kernels this one (the faster of the two for each, as
+// timed on an MI355X).
+template <typename T>
+__device__ __forceinline__ T ldgg(const T *base, int idx)
 {
     return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + (uint32_t)idx * (uint32_t)sizeof(T));
 }
 
 template <typename T>
-__device__ __forceinline__ void stg(T *base, int idx, T v)
+__device__ __forceinline__ void stgg(T *base, int idx, T v)
 {
     *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (uint32_t)idx * (uint32_t)sizeof(T)) = v;
 }
@@ -127,15 +175,18 @@ __device__ __forceinline__ T pmlT_pre(T G, T psi_old, T q, T a)
 template <typename T>
 struct PmlIn {
     T m0, m1, q, a, b;
+    // BUF selects buffer (ldg) or global (ldgg) accesses
+    template <bool BUF>
     __device__ __forceinline__ void load(const T *mem0, const T *mem1, int i, const T *prof, int j, int n)
     {
         const int ic = i >= 0 ? i : 0;
         const int jc = j < 0 ? 0 : (j >= n ? n - 1 : j);
-        m0 = ldg(mem0, ic);
-        m1 = ldg(mem1, ic);
-        q = ldg(prof + 3 * n, jc);
-        a = ldg(prof + 2 * n, jc);
-        b = ldg(prof + n, jc);
+        auto ld = [](const T *b_, int k) { return BUF ? ldg(b_, k) : ldgg(b_, k); };
+        m0 = ld(mem0, ic);
+        m1 = ld(mem1, ic);
+        q = ld(prof + 3 * n, jc);
+        a = ld(prof + 2 * n, jc);
+        b = ld(prof + n, jc);
     }
 };
 
@@ -568,8 +619,8 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
         const int r = r0 - 4 + ty + BLOCK_Y * k;
         zok[k] = r >= 0 && r < nz && c < nx;
         const int p = zok[k] ? r * ld + c : 0;
-        zu[k] = ldg(U, p);
-        zv[k] = ldg(V, p);
+        zu[k] = ldgg(U, p);
+        zv[k] = ldgg(V, p);
     }
     T xu[KX], xv[KX];
     bool xok[KX];
@@ -580,8 +631,8 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
         const int r = r0 - 4 + zi, cc = c0 - 4 + xi;
         xok[k] = ((8 * (TZ + 8)) % 256 == 0 || e < 8 * (TZ + 8)) && r >= 0 && r < nz && cc >= 0 && cc < nx;
         const int p = xok[k] ? r * ld + cc : 0;
-        xu[k] = ldg(U, p);
-        xv[k] = ldg(V, p);
+        xu[k] = ldgg(U, p);
+        xv[k] = ldgg(V, p);
     }
     // owned strains and, in tiles touching a strip, the owned cells' C-PML
     // memories 3 (exx / ezz derivatives) and 4 (exz derivatives)
@@ -593,24 +644,24 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
         const int r = r0 + ty + BLOCK_Y * rr;
         const bool ok = r < nz && c < nx;
         const int q = ok ? r * ld + c : 0;
-        ex0[rr] = ldg((const T *)ex, q);
-        ez0[rr] = ldg((const T *)ez, q);
-        es0[rr] = ldg((const T *)es, q);
+        ex0[rr] = ldgg((const T *)ex, q);
+        ez0[rr] = ldgg((const T *)ez, q);
+        es0[rr] = ldgg((const T *)es, q);
         ixo[rr] = izo[rr] = -1;
         mx3[rr] = mx4[rr] = mz3[rr] = mz4[rr] = T(0);
         if (!inx) {
             ixo[rr] = ok ? xstrip(g, r, c) : -1;
             const T *pb = a.psix + s * a.psix_stride;
             const int i = ixo[rr] >= 0 ? ixo[rr] : 0;
-            mx3[rr] = ldg(pb + 3 * a.psix_n, i);
-            mx4[rr] = ldg(pb + 4 * a.psix_n, i);
+            mx3[rr] = ldgg(pb + 3 * a.psix_n, i);
+            mx4[rr] = ldgg(pb + 4 * a.psix_n, i);
         }
         if (!inz) {
             izo[rr] = ok ? zstrip(g, r, c) : -1;
             const T *pb = a.psiz + s * a.psiz_stride;
             const int i = izo[rr] >= 0 ? izo[rr] : 0;
-            mz3[rr] = ldg(pb + 3 * a.psiz_n, i);
-            mz4[rr] = ldg(pb + 4 * a.psiz_n, i);
+            mz3[rr] = ldgg(pb + 3 * a.psiz_n, i);
+            mz4[rr] = ldgg(pb + 4 * a.psiz_n, i);
         }
     }
 #pragma unroll
@@ -672,8 +723,8 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
                 exn += stm;
                 ezn += stm;
             }
-            stg(ex, q, exn);
-            stg(ez, q, ezn);
+            stgg(ex, q, exn);
+            stgg(ez, q, ezn);
             // exz on [0, nz-2] x [0, nx-2]
             T esn = es0[rr];
             if (CZ(r <= nz - 2) && CX(c <= nx - 2)) {
@@ -687,16 +738,16 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
                 if (!IZ && iz >= 0) dzu = pml_fwd_pre(dzu, mz4[rr], pz + 4 * a.psiz_n, a.profz, r, nz);
                 if (!IX && ix >= 0) dxv = pml_fwd_pre(dxv, mx4[rr], px + 4 * a.psix_n, a.profx, c, nx);
                 esn = esn + T(0.5) * fma(g.dtdz, dzu, g.dtdx * dxv);
-                stg(es, q, esn);
+                stgg(es, q, esn);
             }
             if (!IZ && a.dxu0 && r == 0) a.dxu0[((i64)s * a.nt_max + it) * nx + c] = dxu;
             const int rp = r - g.z0, cp = c - g.pad;
             if (a.hist && CZ(rp >= 0 && rp < g.nzd) && CX(cp >= 0 && cp < g.nxd)) {
                 T *hb = a.hist + ((i64)s * a.nt_max + it) * 3 * g.np;
                 const int o = rp * g.nxd + cp, npi = (int)g.np;
-                stg(hb, o, exn);
-                stg(hb, o + npi, ezn);
-                stg(hb, o + 2 * npi, esn);
+                stgg(hb, o, exn);
+                stgg(hb, o + npi, ezn);
+                stgg(hb, o + 2 * npi, esn);
                 if (a.hist2_mode == 2) {
                     T *h = a.hist2 + ((i64)s * a.nt_max + it) * 2 * g.np + (i64)rp * g.nxd + cp;
                     h[0] = sU[zc][xc];
@@ -930,8 +981,8 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
             const int r = r0 - 4 + ty + BLOCK_Y * k;
             izs[k] = zok[k] ? zstrip(g, r, c) : -1;
             ixs[k] = zok[k] ? xstrip(g, r, c) : -1;
-            pz[k].load(S.pzo, S.pzo + S.pzn, izs[k], a.profz, r, nz);
-            if (k >= 1 && k < TZ / BLOCK_Y + 1) px[k].load(S.pxo, S.pxo + S.pxn, ixs[k], a.profx, c, nx);
+            pz[k].template load<true>(S.pzo, S.pzo + S.pzn, izs[k], a.profz, r, nz);
+            if (k >= 1 && k < TZ / BLOCK_Y + 1) px[k].template load<true>(S.pxo, S.pxo + S.pxn, ixs[k], a.profx, c, nx);
         }
         PmlIn<T> hp[KX];
         int ihs[KX];
@@ -940,7 +991,7 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
             const int e = tid + 256 * k;
             const int zi = e >> 3, h = e & 7, xi = h < 4 ? h : h + TILE_X;
             ihs[k] = hok[k] ? xstrip(g, r0 + zi, c0 - 4 + xi) : -1;
-            hp[k].load(S.pxo, S.pxo + S.pxn, ihs[k], a.profx, c0 - 4 + xi, nx);
+            hp[k].template load<true>(S.pxo, S.pxo + S.pxn, ihs[k], a.profx, c0 - 4 + xi, nx);
         }
         // Dx_U_a input: dt/dx exx_a (columns >= 1) plus, on the free-surface
         // row, the image term L_ratio dz/dx Dz_V_a (row 0 has no z-strip)
@@ -1171,9 +1222,9 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
         const int r = r0 - 4 + ty + BLOCK_Y * k;
         zok[k] = r >= 0 && r < nz && c < nx;
         const int p = zok[k] ? r * ld + c : 0;
-        zb[k] = ldg(Bp, p);
-        zu[k] = ldg(Ua, p);
-        zv[k] = ldg(Va, p);
+        zb[k] = ldgg(Bp, p);
+        zu[k] = ldgg(Ua, p);
+        zv[k] = ldgg(Va, p);
     }
     T hb[KX], hu[KX], hv[KX];
     bool hok[KX];
@@ -1184,9 +1235,9 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
         const int r = r0 + zi, cc = c0 - 4 + xi;
         hok[k] = ((8 * TZ) % 256 == 0 || e < 8 * TZ) && r < nz && cc >= 0 && cc < nx;
         const int p = hok[k] ? r * ld + cc : 0;
-        hb[k] = ldg(Bp, p);
-        hu[k] = ldg(Ua, p);
-        hv[k] = ldg(Va, p);
+        hb[k] = ldgg(Bp, p);
+        hu[k] = ldgg(Ua, p);
+        hv[k] = ldgg(Va, p);
     }
     // owned values and the history frame, loaded ahead of the barrier
     T l2m0[RPT], mu0[RPT], exa0v[RPT], eza0v[RPT], esa0v[RPT], h0[RPT], h1[RPT], h2v[RPT], ac0[RPT], ac1[RPT];
@@ -1196,11 +1247,11 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
     for (int rr = 0; rr < RPT; ++rr) {
         const int r = r0 + ty + BLOCK_Y * rr;
         const int q = (r < nz && c < nx) ? r * ld + c : 0;
-        l2m0[rr] = ldg(a.L2M + g.origin, q);
-        mu0[rr] = ldg(a.M + g.origin, q);
-        exa0v[rr] = ldg((const T *)exa, q);
-        eza0v[rr] = ldg((const T *)eza, q);
-        esa0v[rr] = ldg((const T *)esa, q);
+        l2m0[rr] = ldgg(a.L2M + g.origin, q);
+        mu0[rr] = ldgg(a.M + g.origin, q);
+        exa0v[rr] = ldgg((const T *)exa, q);
+        eza0v[rr] = ldgg((const T *)eza, q);
+        esa0v[rr] = ldgg((const T *)esa, q);
         // history frame: read wherever the imaging below uses it (physical
         // cells; frame -1 is never used); other lanes read entry 0
         h0[rr] = h1[rr] = h2v[rr] = ac0[rr] = ac1[rr] = T(0);
@@ -1210,14 +1261,14 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
             const bool hok = hframe >= 0 && pok;
             const T *hfr = a.hist + (hs + (hframe >= 0 ? hframe : 0)) * 3 * g.np;
             const int o = hok ? rp * g.nxd + cp : 0, npi = (int)g.np;
-            h0[rr] = ldg(hfr, o);
-            h1[rr] = ldg(hfr, o + npi);
-            h2v[rr] = ldg(hfr, o + 2 * npi);
+            h0[rr] = ldgg(hfr, o);
+            h1[rr] = ldgg(hfr, o + npi);
+            h2v[rr] = ldgg(hfr, o + 2 * npi);
             // first two gradient accumulators of the cell
             const T *accb = a.gacc + (i64)s * a.nacc * g.np;
             const int oa = pok ? rp * g.nxd + cp : 0;
-            ac0[rr] = ldg(accb, oa);
-            if (a.gparams >= 2) ac1[rr] = ldg(accb, oa + npi);
+            ac0[rr] = ldgg(accb, oa);
+            if (a.gparams >= 2) ac1[rr] = ldgg(accb, oa + npi);
         }
     }
     // C-PML inputs of the staged cells and the owned cells' G values, kept
@@ -1261,9 +1312,9 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
             const int r = r0 - 4 + ty + BLOCK_Y * k;
             izs[k] = zok[k] ? zstrip(g, r, c) : -1;
             ixs[k] = zok[k] ? xstrip(g, r, c) : -1;
-            pz[k].load(S.pzo + 2 * S.pzn, S.pzo + 3 * S.pzn, izs[k], a.profz, r, nz);
+            pz[k].template load<false>(S.pzo + 2 * S.pzn, S.pzo + 3 * S.pzn, izs[k], a.profz, r, nz);
             if (k >= 1 && k < TZ / BLOCK_Y + 1)
-                px[k].load(S.pxo + 2 * S.pxn, S.pxo + 3 * S.pxn, ixs[k], a.profx, c, nx);
+                px[k].template load<false>(S.pxo + 2 * S.pxn, S.pxo + 3 * S.pxn, ixs[k], a.profx, c, nx);
         }
         PmlIn<T> hp[KX];
         int ihs[KX];
@@ -1272,7 +1323,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
             const int e = tid + 256 * k;
             const int zi = e >> 3, h = e & 7, xi = h < 4 ? h : h + TILE_X;
             ihs[k] = hok[k] ? xstrip(g, r0 + zi, c0 - 4 + xi) : -1;
-            hp[k].load(S.pxo + 2 * S.pxn, S.pxo + 3 * S.pxn, ihs[k], a.profx, c0 - 4 + xi, nx);
+            hp[k].template load<false>(S.pxo + 2 * S.pxn, S.pxo + 3 * S.pxn, ihs[k], a.profx, c0 - 4 + xi, nx);
         }
 #pragma unroll
         for (int k = 0; k < KZ; ++k) {
@@ -1374,9 +1425,9 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
             const T l2m = l2m0[rr], mu = mu0[rr];
             const T lam = l2m - T(2) * mu;
             const T exa0 = exa0v[rr], eza0 = eza0v[rr];
-            stg(exa, q, exa0 + l2m * Ax + lam * Az);
-            stg(eza, q, eza0 + lam * Ax + l2m * Az);
-            stg(esa, q, esa0v[rr] + T(2) * mu * Mes);
+            stgg(exa, q, exa0 + l2m * Ax + lam * Az);
+            stgg(eza, q, eza0 + lam * Ax + l2m * Az);
+            stgg(esa, q, esa0v[rr] + T(2) * mu * Mes);
 
             // C-PML memories of this cell: psi' = b (G + psi), from the values
             // staged for row rr + 1 of the z-slab
