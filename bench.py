@@ -26,6 +26,12 @@ DX = DZ = 20.0
 NSHOT, NREC = 25, 500
 HBM_PEAK = 8.0e12
 
+# HBM bytes per 25-shot launch of the velocity-adjoint kernel by tile height,
+# from rocprofv3 FETCH_SIZE and WRITE_SIZE passes over
+# `bench.py --kernels-only --tiles 8,8,8,4`, scaled by the counters' readings
+# for a 256 MiB device copy (reads x 2.00, writes x 1.00).
+ADJ_VEL_TRAFFIC = {4: (51309.6 * 2.0 + 38341.2) * 1024}
+
 
 def algorithmic_bytes_per_shot_gradient(s=4, k=1, nt=NT):
     """Minimal single-pass traffic: every array touched once per step.
@@ -111,7 +117,9 @@ def main():
     ap.add_argument("--imaging", default="exact", choices=["exact", "zhou"])
     ap.add_argument("--shots", type=int, default=NSHOT,
                     help="shots per gradient (the workload is 25; other values are for experiments)")
-    ap.add_argument("--groups", type=int, default=2, help="concurrent stream groups of shots (1-4)")
+    ap.add_argument("--groups", type=int, default=3, help="concurrent stream groups of shots (1-4)")
+    ap.add_argument("--tiles", default=None,
+                    help="fixed tile heights of the four time-step kernels, e.g. 8,8,8,4 (default: autotune)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="after one forward, time each time-step kernel (20 launches) and a "
                          "256 MiB device copy; for hardware-counter runs")
@@ -134,7 +142,11 @@ def main():
         # no observed data: time each time-step kernel on the plan's buffers
         plan.set_model(*(torch.as_tensor(a, dtype=f32, device=dev) for a in sv["init"]))
         plan.set_shots(sv["src_x"], sv["src_z"], "moment", NT, sv["rec_x"], sv["rec_z"], sv["wav"])
-        tiles, _ = plan.autotune(10)
+        if args.tiles:
+            tiles = [int(t) for t in args.tiles.split(",")]
+            plan.set_tiles(tiles)
+        else:
+            tiles, _ = plan.autotune(10)
         res = {n: plan.time_kernel(i, 20) for i, n in enumerate(("fwd_vel", "fwd_strain", "adj_strain", "adj_vel"))}
         a = torch.zeros(64 << 20, dtype=f32, device=dev)
         b = torch.empty_like(a)
@@ -150,7 +162,11 @@ def main():
     scale = float(1.0 / (torch.mean(obs.double() ** 2) + 1e-45))
     vs0, vp0, rho0 = (torch.as_tensor(a, dtype=f32, device=dev) for a in sv["init"])
     plan.set_model(vs0, vp0, rho0)
-    tiles, tune = plan.autotune(20)
+    if args.tiles:
+        tiles, tune = [int(t) for t in args.tiles.split(",")], None
+        plan.set_tiles(tiles)
+    else:
+        tiles, tune = plan.autotune(20)
 
     def step():
         plan.set_model(vs0, vp0, rho0)
@@ -178,7 +194,8 @@ def main():
     avg_s = per_kernel["adj_vel"] * 1e-6
     kb = dominant_kernel_bytes(4, 1)
     kern = {"bound": "hbm", "achieved": kb / avg_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": kb / avg_s / HBM_PEAK, "traffic": None, "kernel": f"k_adj_vel<float,8,{tiles[3]}>",
+            "frac": kb / avg_s / HBM_PEAK, "traffic": ADJ_VEL_TRAFFIC.get(tiles[3]),
+            "kernel": f"k_adj_vel<float,8,{tiles[3]}>",
             "avg_launch_us": avg_s * 1e6, "algorithmic_bytes_per_launch": kb}
     tot, _, _ = algorithmic_bytes_per_shot_gradient(4, 1)
     path = {"algorithmic_GB_per_shot_gradient": tot / 1e9, "achieved_GB_s": value * tot / 1e9,
