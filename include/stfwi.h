@@ -204,6 +204,38 @@ int stfwi_set_tiles(stfwi_plan *plan, const int *tz);
  * back into the caller's stream; results do not depend on the grouping. */
 int stfwi_set_groups(stfwi_plan *plan, int groups);
 
+/* ---- L-BFGS in float64 on the device (Nocedal & Wright Alg. 7.4).  The
+ * reference runs its quasi-Newton step on the host: scipy L-BFGS-B in
+ * run_multiscale_fwi (fwi/optim_lbfgsb.py:409-434) and jaxopt.LBFGS in
+ * run_lbfgs (fwi/optimizer.py:110-117).  All vectors are device float64;
+ * each call is one workgroup with a fixed summation order (bitwise
+ * reproducible). ---- */
+
+/* d = -H g from m stored pairs: rows slots[0..m) (oldest first) of S and Y
+ * (row stride ld), rho[k] = 1/(y.s) of row slots[k] (host arrays, m <= 64),
+ * H0 = gamma I.  *gd (device scalar, may be NULL) receives g . d. */
+int stfwi_lbfgs_direction(const double *g, const double *S, const double *Y, const double *rho, const int *slots,
+                          int m, long long n, long long ld, double gamma, double *d, double *gd, void *stream);
+
+/* s = x_new - x_old, y = g_new - g_old into s_out / y_out; dots (device, 3)
+ * receives y.s, y.y, s.s. */
+int stfwi_lbfgs_pair(const double *x_new, const double *x_old, const double *g_new, const double *g_old,
+                     long long n, double *s_out, double *y_out, double *dots, void *stream);
+
+/* *out = a . b (device scalar). */
+int stfwi_dot64(const double *a, const double *b, long long n, double *out, void *stream);
+
+/* out = clip(x + t d, lo, hi) elementwise (the projection onto per-cell
+ * bounds of fwi/optim_lbfgsb.py:298-309); lo / hi may be NULL. */
+int stfwi_step_clip(const double *x, const double *d, double t, const double *lo, const double *hi, long long n,
+                    double *out, void *stream);
+
+/* v[i] = 0 where x[i] sits on a bound and g[i] points out of the box
+ * (x <= lo, g > 0 or x >= hi, g < 0): the binding set masked out of the
+ * gradient and the search direction of the projected method. */
+int stfwi_bound_mask(const double *x, const double *g, const double *lo, const double *hi, long long n, double *v,
+                     void *stream);
+
 #ifdef __cplusplus
 }
 #endif

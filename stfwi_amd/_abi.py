@@ -45,6 +45,12 @@ SIGNATURES = {
                                      C.c_double, C.c_double, C.c_double, C.c_int, _VP, _VP]),
     "stfwi_residual": (C.c_int, [_VP, _VP, _VP, C.c_int, _VP, C.c_int, C.c_int, C.c_double, C.c_int,
                                  _VP, _VP, _VP]),
+    "stfwi_lbfgs_direction": (C.c_int, [_VP, _VP, _VP, _VP, _VP, C.c_int, C.c_longlong, C.c_longlong,
+                                        C.c_double, _VP, _VP, _VP]),
+    "stfwi_lbfgs_pair": (C.c_int, [_VP, _VP, _VP, _VP, C.c_longlong, _VP, _VP, _VP, _VP]),
+    "stfwi_dot64": (C.c_int, [_VP, _VP, C.c_longlong, _VP, _VP]),
+    "stfwi_step_clip": (C.c_int, [_VP, _VP, C.c_double, _VP, _VP, C.c_longlong, _VP, _VP]),
+    "stfwi_bound_mask": (C.c_int, [_VP, _VP, _VP, _VP, C.c_longlong, _VP, _VP]),
 }
 
 

@@ -15,6 +15,7 @@
 #include "../../include/stfwi.h"
 #include "fwi_kernels.h"
 #include "post_kernels.h"
+#include "lbfgs_kernels.h"
 
 using namespace stfwi;
 
@@ -855,6 +856,69 @@ extern "C" int stfwi_zhou_imaging(const void *ex, const void *ez, const void *es
                            (const float *)es, (const float *)ua, (const float *)va, (const float *)b_full,
                            (const float *)vx, (const float *)vz, nt, nz, nx, (float)dx, (float)dz, (float)dt,
                            (float *)out);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// L-BFGS (float64)
+// ---------------------------------------------------------------------------
+extern "C" int stfwi_lbfgs_direction(const double *g, const double *S, const double *Y, const double *rho,
+                                     const int *slots, int m, long long n, long long ld, double gamma, double *d,
+                                     double *gd, void *stream)
+{
+    if (!g || !d || n < 1 || m < 0 || m > LB_MAX_M || (m > 0 && (!S || !Y || !rho || !slots || ld < n)))
+        return fail(STFWI_ERR_INVALID, "bad argument (m=%d, n=%lld, ld=%lld)", m, n, ld);
+    if (!(gamma > 0)) return fail(STFWI_ERR_INVALID, "gamma must be > 0");
+    LbHist h;
+    memset(&h, 0, sizeof h);
+    for (int k = 0; k < m; ++k) {
+        if (slots[k] < 0) return fail(STFWI_ERR_INVALID, "negative history slot");
+        h.slot[k] = slots[k];
+        h.rho[k] = rho[k];
+    }
+    hipLaunchKernelGGL(k_lbfgs_dir, dim3(1), dim3(LB_THREADS), 0, (hipStream_t)stream, g, S, Y, h, m, n, ld, gamma,
+                       d, gd);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int stfwi_lbfgs_pair(const double *x_new, const double *x_old, const double *g_new, const double *g_old,
+                                long long n, double *s_out, double *y_out, double *dots, void *stream)
+{
+    if (!x_new || !x_old || !g_new || !g_old || !s_out || !y_out || !dots || n < 1)
+        return fail(STFWI_ERR_INVALID, "bad argument");
+    hipLaunchKernelGGL(k_lbfgs_pair, dim3(1), dim3(LB_THREADS), 0, (hipStream_t)stream, x_new, x_old, g_new, g_old,
+                       n, s_out, y_out, dots);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int stfwi_dot64(const double *a, const double *b, long long n, double *out, void *stream)
+{
+    if (!a || !b || !out || n < 1) return fail(STFWI_ERR_INVALID, "bad argument");
+    hipLaunchKernelGGL(k_dot64, dim3(1), dim3(LB_THREADS), 0, (hipStream_t)stream, a, b, n, out);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int stfwi_step_clip(const double *x, const double *d, double t, const double *lo, const double *hi,
+                               long long n, double *out, void *stream)
+{
+    if (!x || !d || !out || n < 1) return fail(STFWI_ERR_INVALID, "bad argument");
+    const long long blocks = std::min<long long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_step_clip, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, x, d, t, lo, hi, n,
+                       out);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int stfwi_bound_mask(const double *x, const double *g, const double *lo, const double *hi, long long n,
+                                double *v, void *stream)
+{
+    if (!x || !g || !v || n < 1) return fail(STFWI_ERR_INVALID, "bad argument");
+    const long long blocks = std::min<long long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_bound_mask, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, x, g, lo, hi, n, v);
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -8,3 +8,5 @@ from .preconditioning import eprecond3, eprecond3_preconditioner  # noqa: F401
 from .illumination import compute_illumination_for_stage  # noqa: F401
 from .loss import build_loss_fn, gaussian_smooth_2d  # noqa: F401
 from .filters import jax_biquad, jax_sosfilt, cosine_taper_end, sosfilt_T  # noqa: F401
+from .projections import project_velocity, cfl_vp_cap, k_safe_ratio  # noqa: F401
+from .optim_lbfgsb import run_multiscale_fwi  # noqa: F401

@@ -1,0 +1,222 @@
+"""The plain method's L-BFGS step (float64 device kernels) and the
+multiscale driver: the two-loop direction against the dense BFGS inverse
+Hessian, convergence against scipy's L-BFGS-B on convex problems, the
+projections and the driver's stage bookkeeping."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import lbfgs as olb
+
+
+def _pairs(n, m, seed):
+    """m curvature pairs of an SPD quadratic (y = A s, so y.s > 0)."""
+    rng = np.random.default_rng(seed)
+    Q = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    A = Q @ np.diag(np.linspace(0.5, 20.0, n)) @ Q.T
+    S = [rng.standard_normal(n) for _ in range(m)]
+    Y = [A @ s for s in S]
+    g = rng.standard_normal(n)
+    gamma = float(np.dot(Y[-1], S[-1]) / np.dot(Y[-1], Y[-1]))
+    return A, S, Y, g, gamma
+
+
+@pytest.mark.parametrize("m", [0, 1, 5, 10])
+def test_oracle_two_loop_equals_dense_bfgs(m):
+    n = 60
+    _, S, Y, g, gamma = _pairs(n, max(m, 1), 1)
+    S, Y = S[:m], Y[:m]
+    gamma = gamma if m else 1.0
+    d = olb.two_loop(g, S, Y, gamma)
+    H = olb.dense_inverse_hessian(S, Y, gamma, n)
+    np.testing.assert_allclose(d, -H @ g, rtol=1e-11, atol=1e-12 * np.abs(H @ g).max())
+    # the dense update satisfies the secant equation of the newest pair
+    if m:
+        np.testing.assert_allclose(H @ Y[-1], S[-1], rtol=1e-9, atol=1e-10)
+
+
+def test_projections_closed_form():
+    from stfwi_amd.fwi.projections import project_velocity, k_safe_ratio, cfl_vp_cap
+    rng = np.random.default_rng(0)
+    vp = rng.uniform(1500, 5000, (20, 30))
+    vs = vp / rng.uniform(1.0, 2.0, vp.shape)
+    rho = rng.uniform(-1, 2500, vp.shape)
+    cap = cfl_vp_cap(10.0, 12.0, 1e-3, 0.5)
+    assert cap == 0.5 * 10.0 / 1e-3
+    vs2, vp2, rho2 = project_velocity(vs, vp, rho, K_margin=0.04, vp_cap=cap)
+    r = k_safe_ratio(0.04)
+    assert abs(r - 1.04 * 2 / np.sqrt(3)) < 1e-15
+    assert np.all(vp2 <= cap) and np.all(vp2 == np.minimum(vp, cap))
+    assert np.all(vs2 <= vp2 / r + 1e-12)
+    assert np.all(vs2 == np.maximum(np.minimum(vs, vp2 / r), 1e-3))
+    assert np.all(rho2 >= 1e-3)
+    # K = rho (Vp^2 - 4/3 Vs^2) > 0 after projection
+    assert np.all(vp2 ** 2 - 4.0 / 3.0 * vs2 ** 2 > 0)
+
+
+def test_plateau_detector():
+    from stfwi_amd.fwi.optim_lbfgsb import _Plateau, _active_keys
+    p = _Plateau(2, 0.1)
+    assert not any(p.push(v) for v in (10.0, 9.0, 5.0))
+    # mean(last 2) = (5 + 4.9)/2 >= 0.9 * mean(10, 9)?  4.95 < 8.55 -> keep going
+    assert not p.push(4.9)
+    assert not p.push(4.9)      # mean(4.9, 4.9) = 4.9 < 0.9 * mean(9, 5) = 6.3
+    assert p.push(4.9)          # mean(4.9, 4.9) >= 0.9 * mean(5, 4.9)
+    assert not _Plateau(None, 0.1).push(1.0)
+    assert _active_keys("vs_vp") == ["Vs", "Vp"]
+    with pytest.raises(ValueError):
+        _active_keys("vp")
+
+
+# ---------------------------------------------------------------- GPU ----
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("m", [1, 4, 10])
+def test_gpu_two_loop_matches_dense_bfgs(m):
+    import torch
+    from stfwi_amd.lbfgs import LBFGSMemory
+    n = 300
+    A, S, Y, g, gamma = _pairs(n, m, 2)
+    dev = torch.device("cuda", 0)
+    mem = LBFGSMemory(n, m=m, device=dev)
+    x = torch.zeros(n, dtype=torch.float64, device=dev)
+    gz = torch.zeros(n, dtype=torch.float64, device=dev)
+    for s, y in zip(S, Y):
+        assert mem.push(torch.as_tensor(s, device=dev), x, torch.as_tensor(y, device=dev), gz)
+    assert abs(mem.gamma - gamma) < 1e-15 * gamma
+    gt = torch.as_tensor(g, device=dev)
+    d, gd = mem.direction(gt)
+    d = d.cpu().numpy()
+    ref = -olb.dense_inverse_hessian(S, Y, gamma, n) @ g
+    assert np.linalg.norm(d - ref) / np.linalg.norm(ref) < 1e-13
+    assert np.linalg.norm(d - olb.two_loop(g, S, Y, gamma)) / np.linalg.norm(ref) < 1e-14
+    assert abs(gd - float(np.dot(g, d))) < 1e-12 * abs(gd)
+
+
+@pytest.mark.gpu
+def test_gpu_history_keeps_the_newest_pairs():
+    """With more pairs than rows the oldest drop out, and a pair failing the
+    curvature test leaves the stored ones untouched."""
+    import torch
+    from stfwi_amd.lbfgs import LBFGSMemory
+    n, m = 200, 3
+    A, S, Y, g, _ = _pairs(n, 6, 3)
+    dev = torch.device("cuda", 0)
+    mem = LBFGSMemory(n, m=m, device=dev)
+    z = torch.zeros(n, dtype=torch.float64, device=dev)
+    for s, y in zip(S, Y):
+        mem.push(torch.as_tensor(s, device=dev), z, torch.as_tensor(y, device=dev), z)
+    bad = torch.as_tensor(-S[0], device=dev)
+    assert not mem.push(bad, z, torch.as_tensor(Y[0], device=dev), z)   # y.s < 0
+    gamma = float(np.dot(Y[-1], S[-1]) / np.dot(Y[-1], Y[-1]))
+    d, _ = mem.direction(torch.as_tensor(g, device=dev))
+    ref = -olb.dense_inverse_hessian(S[-m:], Y[-m:], gamma, n) @ g
+    assert np.linalg.norm(d.cpu().numpy() - ref) / np.linalg.norm(ref) < 1e-13
+
+
+def _quadratic(n, seed, dev):
+    import torch
+    rng = np.random.default_rng(seed)
+    Q = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    A = Q @ np.diag(np.geomspace(1.0, 100.0, n)) @ Q.T
+    b = rng.standard_normal(n)
+    At = torch.as_tensor(A, device=dev)
+    bt = torch.as_tensor(b, device=dev)
+
+    def fun(x):
+        Ax = At @ x
+        return float(0.5 * (x @ Ax) - bt @ x), Ax - bt
+    return A, b, fun
+
+
+@pytest.mark.gpu
+def test_gpu_minimize_quadratic_reaches_solution_like_scipy():
+    import torch
+    from scipy.optimize import minimize as sp_min
+    from stfwi_amd.lbfgs import minimize
+    dev = torch.device("cuda", 0)
+    n = 400
+    A, b, fun = _quadratic(n, 4, dev)
+    xs = np.linalg.solve(A, b)
+    res = minimize(fun, torch.zeros(n, dtype=torch.float64, device=dev), maxiter=300)
+    x = res["x"].cpu().numpy()
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-6
+    sp = sp_min(lambda v: (0.5 * v @ A @ v - b @ v, A @ v - b), np.zeros(n), jac=True, method="L-BFGS-B",
+                options=dict(maxiter=300, gtol=0.0, ftol=1e-30, maxcor=10))
+    assert np.linalg.norm(x - sp.x) / np.linalg.norm(xs) < 1e-5
+    assert res["fun"] <= sp.fun + 1e-9 * abs(sp.fun)
+
+
+@pytest.mark.gpu
+def test_gpu_minimize_with_bounds_matches_scipy_minimiser():
+    """Convex quadratic with box constraints: the minimiser is unique, so the
+    projected L-BFGS and scipy's L-BFGS-B reach the same point."""
+    import torch
+    from scipy.optimize import minimize as sp_min
+    from stfwi_amd.lbfgs import minimize
+    dev = torch.device("cuda", 0)
+    n = 200
+    A, b, fun = _quadratic(n, 5, dev)
+    lo, hi = -0.05, 0.05
+    res = minimize(fun, torch.zeros(n, dtype=torch.float64, device=dev),
+                   lo=torch.full((n,), lo, dtype=torch.float64, device=dev),
+                   hi=torch.full((n,), hi, dtype=torch.float64, device=dev), maxiter=2000)
+    sp = sp_min(lambda v: (0.5 * v @ A @ v - b @ v, A @ v - b), np.zeros(n), jac=True, method="L-BFGS-B",
+                bounds=[(lo, hi)] * n, options=dict(maxiter=2000, gtol=1e-12, ftol=1e-30))
+    x = res["x"].cpu().numpy()
+    assert np.all(x >= lo - 1e-15) and np.all(x <= hi + 1e-15)
+    assert abs(res["fun"] - sp.fun) < 1e-6 * abs(sp.fun)
+    assert np.linalg.norm(x - sp.x) / np.linalg.norm(sp.x) < 1e-3
+
+
+@pytest.mark.gpu
+def test_gpu_minimize_rosenbrock():
+    import torch
+    from stfwi_amd.lbfgs import minimize
+    dev = torch.device("cuda", 0)
+    n = 50
+
+    def fun(x):
+        a, bb = x[:-1], x[1:]
+        f = float((100.0 * (bb - a * a) ** 2 + (1 - a) ** 2).sum())
+        g = torch.zeros_like(x)
+        g[:-1] += -400.0 * a * (bb - a * a) - 2.0 * (1 - a)
+        g[1:] += 200.0 * (bb - a * a)
+        return f, g
+    res = minimize(fun, torch.full((n,), -1.0, dtype=torch.float64, device=dev), maxiter=2000)
+    assert res["fun"] < 1e-10
+    assert torch.allclose(res["x"], torch.ones_like(res["x"]), atol=1e-5)
+
+
+@pytest.mark.gpu
+def test_gpu_multiscale_driver_decreases_misfit_and_writes_checkpoints(tmp_path):
+    import torch
+    from stfwi_amd.fwi import forward_jax, ricker_jax
+    from stfwi_amd.fwi.optim_lbfgsb import run_multiscale_fwi
+    from stfwi_amd.models import anomaly_model
+    nz, nx, pad, nt, dt, dx = 40, 64, 10, 300, 1e-3, 10.0
+    vs_t, vp_t, rho_t = anomaly_model(nz, nx, seed=1)
+    vs0 = np.full_like(vs_t, 1730.0)
+    sx, sz = np.array([10, 32, 54]), np.array([2, 2, 2])
+    rx = np.arange(0, nx, 2)
+    rz = np.full_like(rx, 1)
+    wav = ricker_jax(torch.arange(nt, dtype=torch.float64) * dt, 15.0, 1.5 / 15.0)
+    obs = np.stack([forward_jax(vs_t, vp_t, rho_t, int(a), int(b), rx, rz, nx, nz, dx, dx, dt, nt, 15.0, pad, wav,
+                                fd_order=4, free_surface=True)[2].cpu().numpy() for a, b in zip(sx, sz)])
+    stages = [dict(f_c=15.0, n_iters=3, invert="vs", illumination_eps=1e-3, grad_smooth_sigma=1.0)]
+    params, hist = run_multiscale_fwi(
+        Vs_init=vs0, Vp_init=vp_t, rho_init=rho_t, src_x=sx, src_z=sz, rec_x=rx, rec_z=rz, nx_dom=nx, nz_dom=nz,
+        dx=dx, dz=dx, dt=dt, nt=nt, pad=pad, observed=obs, stages=stages, component=2, backend="ad", fd_order=4,
+        free_surface=True, vs_bounds=(1000.0, 2500.0), save_dir=str(tmp_path), save_every=1, verbose=False)
+    lh = hist[0]["loss_history"]
+    assert hist[0]["n_evals"] >= 4 and lh.min() < 0.9 * lh[0]
+    assert np.sqrt(np.mean((params[0] - vs_t) ** 2)) < np.sqrt(np.mean((vs0 - vs_t) ** 2))
+    assert np.all(params[1] == vp_t)
+    sd = tmp_path / "stage1"
+    for f in ("epoch_0000.npz", "epoch_0001.npz", "eval_0001.npz", "params.npz"):
+        assert (sd / f).exists(), f
+    with np.load(sd / "params.npz", allow_pickle=True) as z:
+        assert {"Vs", "Vp", "rho", "loss_history", "f_c", "cutoff", "invert", "backend"} <= set(z.files)
+    with np.load(sd / "epoch_0001.npz") as z:
+        assert {"Vs", "Vp", "rho", "iter", "loss"} <= set(z.files)
