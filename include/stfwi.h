@@ -59,6 +59,13 @@ typedef struct stfwi_config {
     int imaging;          /* STFWI_IMAGING_* (history storage is sized for it) */
     int grad_params;      /* STFWI_GRAD_* */
     int store_history;    /* 1: keep the strain history for the adjoint */
+    int ckpt_every;       /* 0: history of every step; K > 0: checkpoint the
+                           * forward state every K steps and recompute each
+                           * K-step segment's history during the adjoint
+                           * (memory: nt/K states + K+1 history frames; the
+                           * reference's jax.checkpoint blocks,
+                           * fwi/forward.py:471-501).  Results do not depend
+                           * on K. */
 } stfwi_config;
 
 /* One shot.  Indices are physical-domain (the reference's src_x, src_z,

@@ -59,7 +59,7 @@ class Config(C.Structure):
                 ("fd_order", C.c_int), ("dx", C.c_double), ("dz", C.c_double), ("dt", C.c_double),
                 ("fc", C.c_double), ("dtype", C.c_int), ("n_shots_max", C.c_int), ("nt_max", C.c_int),
                 ("nrec_max", C.c_int), ("imaging", C.c_int), ("grad_params", C.c_int),
-                ("store_history", C.c_int)]
+                ("store_history", C.c_int), ("ckpt_every", C.c_int)]
 
 
 class Shot(C.Structure):

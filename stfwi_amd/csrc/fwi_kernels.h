@@ -303,8 +303,9 @@ struct FwdArgs {
     T *rec;                          // [shot][4][nt_max][nrec_max]
     int nrec_max, rec_mask;
     const i64 *rec_cell;             // cell offset r*ld + c of every receiver
-    T *hist;                         // [shot][nt_max][3][np] strains
-    T *hist2;                        // [shot][nt_max][2][np]: 1 = dU/dB,dV/dB  2 = U,V
+    T *hist;                         // [shot][hist_nt][3][np] strains
+    T *hist2;                        // [shot][hist_nt][2][np]: 1 = dU/dB,dV/dB  2 = U,V
+    int hist_nt, hist_t0;            // frames per shot; step of frame 0
     int hist2_mode;
     T *dxu0;                         // [shot][nt_max][nx]
     T *illum;                        // [shot][nz*nx]
@@ -564,7 +565,7 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
             if (a.hist2_mode == 1) {
                 const int rp = r - g.z0, cp = c - g.pad;
                 if (CZ(rp >= 0 && rp < g.nzd) && CX(cp >= 0 && cp < g.nxd)) {
-                    T *h = a.hist2 + ((i64)s * a.nt_max + it) * 2 * g.np + (i64)rp * g.nxd + cp;
+                    T *h = a.hist2 + ((i64)s * a.hist_nt + (it - a.hist_t0)) * 2 * g.np + (i64)rp * g.nxd + cp;
                     h[0] = fma(g.dtdz, D3, fma(g.dtdx, D2, g.dtdx * D1));
                     h[g.np] = fma(g.dtdz, D6, fma(g.dtdz, D5, g.dtdx * D4)) + sv;
                 }
@@ -743,13 +744,13 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
             if (!IZ && a.dxu0 && r == 0) a.dxu0[((i64)s * a.nt_max + it) * nx + c] = dxu;
             const int rp = r - g.z0, cp = c - g.pad;
             if (a.hist && CZ(rp >= 0 && rp < g.nzd) && CX(cp >= 0 && cp < g.nxd)) {
-                T *hb = a.hist + ((i64)s * a.nt_max + it) * 3 * g.np;
+                T *hb = a.hist + ((i64)s * a.hist_nt + (it - a.hist_t0)) * 3 * g.np;
                 const int o = rp * g.nxd + cp, npi = (int)g.np;
                 stgg(hb, o, exn);
                 stgg(hb, o + npi, ezn);
                 stgg(hb, o + 2 * npi, esn);
                 if (a.hist2_mode == 2) {
-                    T *h = a.hist2 + ((i64)s * a.nt_max + it) * 2 * g.np + (i64)rp * g.nxd + cp;
+                    T *h = a.hist2 + ((i64)s * a.hist_nt + (it - a.hist_t0)) * 2 * g.np + (i64)rp * g.nxd + cp;
                     h[0] = sU[zc][xc];
                     h[g.np] = sV[zc][xc];
                 }
@@ -797,7 +798,8 @@ struct AdjArgs {
     const int *uc_ptr, *uc_rec;        // CSR of receiver indices per merged cell
     T *src_adj;                        // [shot][nt_max]
     int imaging, gparams, nacc;
-    const T *hist, *hist2;
+    const T *hist, *hist2;             // [shot][hist_nt][3 | 2][np]
+    int hist_nt, hist_t0;              // frames per shot; step of frame 0
     const T *dxu0;
     T *gacc;                           // [shot][nacc][np]
     T *snap;
@@ -1241,7 +1243,8 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
     }
     // owned values and the history frame, loaded ahead of the barrier
     T l2m0[RPT], mu0[RPT], exa0v[RPT], eza0v[RPT], esa0v[RPT], h0[RPT], h1[RPT], h2v[RPT], ac0[RPT], ac1[RPT];
-    const i64 hs = (i64)s * a.nt_max;
+    const i64 hs = (i64)s * a.nt_max;            // wavelet / dxu0 rows
+    const i64 hh = (i64)s * a.hist_nt - a.hist_t0; // history frame base
     const int hframe = a.imaging == 2 ? it - 1 : it;
 #pragma unroll
     for (int rr = 0; rr < RPT; ++rr) {
@@ -1259,7 +1262,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
             const int rp = r - g.z0, cp = c - g.pad;
             const bool pok = rp >= 0 && rp < g.nzd && cp >= 0 && cp < g.nxd;
             const bool hok = hframe >= 0 && pok;
-            const T *hfr = a.hist + (hs + (hframe >= 0 ? hframe : 0)) * 3 * g.np;
+            const T *hfr = a.hist + (hh + (hframe >= a.hist_t0 ? hframe : a.hist_t0)) * 3 * g.np;
             const int o = hok ? rp * g.nxd + cp : 0, npi = (int)g.np;
             h0[rr] = ldgg(hfr, o);
             h1[rr] = ldgg(hfr, o + npi);
@@ -1489,7 +1492,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
                     gm += Qm;
                 }
                 if (a.gparams == 3) {
-                    const T *h2 = a.hist2 + (hs + it) * 2 * g.np + ip;
+                    const T *h2 = a.hist2 + (hh + it) * 2 * g.np + ip;
                     gb = fma(Ua[q], h2[0], Va[q] * h2[g.np]);
                     acc[0] = ac0[rr] + gl2m;
                     acc[g.np] = ac1[rr] + gl;
@@ -1521,11 +1524,11 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
                     T grl = 0;
                     const T i2dt = T(2) * g.dt;
                     if (it >= 2 && it <= sh.nt - 1) {
-                        const T *h2 = a.hist2 + (hs + it - 1) * 2 * g.np + ip;
+                        const T *h2 = a.hist2 + (hh + it - 1) * 2 * g.np + ip;
                         grl += Ua[q] * h2[0] / i2dt + Va[q] * h2[g.np] / i2dt;
                     }
                     if (it <= sh.nt - 3) {
-                        const T *h2 = a.hist2 + (hs + it + 1) * 2 * g.np + ip;
+                        const T *h2 = a.hist2 + (hh + it + 1) * 2 * g.np + ip;
                         grl -= Ua[q] * h2[0] / i2dt + Va[q] * h2[g.np] / i2dt;
                     }
                     acc[0] = ac0[rr] + glam;
@@ -1541,6 +1544,25 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
     STFWI_DISPATCH(inx, inz, body);
 }
 
+
+// history frame `frame` of every shot from the strains held in the state
+// buffers (the state restored from a checkpoint is the carry-in of the
+// segment's first step)
+template <typename T>
+__global__ void k_hist_from_state(Geo<T> g, const T *state, i64 sstride, T *hist, int hist_nt, int frame, int shot0)
+{
+    const int s = shot0 + blockIdx.y;
+    const i64 np = g.np;
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < np; i += (i64)gridDim.x * blockDim.x) {
+        const int rp = (int)(i / g.nxd), cp = (int)(i - (i64)rp * g.nxd);
+        const i64 cell = g.origin + (i64)(rp + g.z0) * g.ld + cp + g.pad;
+        const T *st = state + s * sstride;
+        T *h = hist + ((i64)s * hist_nt + frame) * 3 * np + i;
+        h[0] = st[2 * g.fstride + cell];
+        h[np] = st[3 * g.fstride + cell];
+        h[2 * np] = st[4 * g.fstride + cell];
+    }
+}
 
 // sum of per-shot accumulators and chain rule to (Vs, Vp, rho)
 template <typename T, typename C>

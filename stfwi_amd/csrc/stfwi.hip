@@ -58,6 +58,14 @@ struct stfwi_plan {
     void *hist = nullptr, *hist2 = nullptr, *dxu0 = nullptr;
     void *gacc = nullptr;
     int hist2_mode = 0;
+    // checkpointed recomputation: every `ck` steps the forward state (5 fields
+    // + C-PML memories) is saved into slot it/ck of `ckpt`; the history then
+    // holds ck + 1 frames (one segment plus its carry-in) and the adjoint
+    // recomputes each earlier segment from its checkpoint.  `state_a` is the
+    // adjoint's own state buffer in that mode.
+    int ck = 0, hist_nt = 0;
+    void *ckpt = nullptr, *state_a = nullptr;
+    i64 ck_state = 0, ck_psix = 0, ck_psiz = 0;   // elements per slot region
     int tz[4] = {16, 16, 16, 16};   // tile height per time-step kernel (autotuned)
     // shots are split into `groups` contiguous groups whose time loops run on
     // their own streams, so one group's launches fill the other's tails
@@ -151,6 +159,10 @@ extern "C" int stfwi_plan_create(const stfwi_config *cfg, stfwi_plan **out)
     if (c.imaging < 0 || c.imaging > 2 || c.grad_params < 1 || c.grad_params > 3)
         return fail(STFWI_ERR_INVALID, "imaging=%d grad_params=%d", c.imaging, c.grad_params);
     if (c.dx <= 0 || c.dz <= 0 || c.dt <= 0) return fail(STFWI_ERR_INVALID, "dx, dz, dt must be positive");
+    if (c.ckpt_every < 0) return fail(STFWI_ERR_INVALID, "ckpt_every=%d", c.ckpt_every);
+    if (c.ckpt_every > 0 && c.imaging == STFWI_IMAGING_ZHOU && c.grad_params == STFWI_GRAD_ALL)
+        return fail(STFWI_ERR_UNSUPPORTED, "checkpointing does not support Zhou imaging of rho (it pairs steps "
+                                           "across segment ends)");
     stfwi_plan *p = new stfwi_plan();
     p->cfg = c;
     p->esz = c.dtype == STFWI_F64 ? 8 : 4;
@@ -185,10 +197,20 @@ extern "C" int stfwi_plan_create(const stfwi_config *cfg, stfwi_plan **out)
         rc |= dalloc(p, &p->psiz_a[k], S * 4 * p->psiz_len * es);
     }
     if (c.store_history && c.imaging) {
-        rc |= dalloc(p, &p->hist, (size_t)S * c.nt_max * 3 * p->np * es);
+        p->ck = c.ckpt_every > 0 && c.ckpt_every < c.nt_max ? c.ckpt_every : 0;
+        p->hist_nt = p->ck ? p->ck + 1 : c.nt_max;
+        rc |= dalloc(p, &p->hist, (size_t)S * p->hist_nt * 3 * p->np * es);
         if (c.grad_params == 3) {
             p->hist2_mode = c.imaging == 2 ? 1 : 2;
-            rc |= dalloc(p, &p->hist2, (size_t)S * c.nt_max * 2 * p->np * es);
+            rc |= dalloc(p, &p->hist2, (size_t)S * p->hist_nt * 2 * p->np * es);
+        }
+        if (p->ck) {
+            const i64 nslot = (c.nt_max + p->ck - 1) / p->ck;
+            p->ck_state = S * 5 * p->fstride;
+            p->ck_psix = S * 5 * p->psix_len;
+            p->ck_psiz = S * 5 * p->psiz_len;
+            rc |= dalloc(p, &p->ckpt, (size_t)nslot * (p->ck_state + p->ck_psix + p->ck_psiz) * es);
+            rc |= dalloc(p, &p->state_a, S * 5 * p->fstride * es);
         }
         if (c.imaging == 2 && c.free_surface) rc |= dalloc(p, &p->dxu0, (size_t)S * c.nt_max * p->nx * es);
         rc |= dalloc(p, &p->gacc, (size_t)S * p->nacc * p->np * es);
@@ -426,6 +448,7 @@ static FwdArgs<T> fwd_args(stfwi_plan *p, void *rec, int rec_mask, void *illum, 
     a.rec_cell = p->rec_cell_d;
     a.hist = (T *)p->hist;
     a.hist2 = (T *)p->hist2; a.hist2_mode = p->hist ? p->hist2_mode : 0;
+    a.hist_nt = p->hist_nt; a.hist_t0 = 0;
     a.dxu0 = (T *)p->dxu0;
     a.illum = (T *)illum; a.illum_comp = illum_comp;
     a.snap = (T *)snap; a.snap_mask = snap_mask; a.nsnap = popcount5(snap_mask);
@@ -455,10 +478,31 @@ static AdjArgs<T> adj_args(stfwi_plan *p, const void *rec_adj, void *src_adj, vo
     a.imaging = p->hist ? c.imaging : 0;
     a.gparams = c.grad_params; a.nacc = p->nacc;
     a.hist = (const T *)p->hist; a.hist2 = (const T *)p->hist2; a.dxu0 = (const T *)p->dxu0;
+    a.hist_nt = p->hist_nt; a.hist_t0 = 0;
+    if (p->ck) a.state = (T *)p->state_a;
     a.gacc = (T *)p->gacc;
     a.snap = (T *)snap; a.snap_mask = snap_mask; a.nsnap = popcount5(snap_mask);
     a.tiles_x = (p->nx + TILE_X - 1) / TILE_X;
     return a;
+}
+
+// checkpoint slot k <- (restore = false) or -> (restore = true) the forward
+// state and C-PML memories of shots [s0, s1), on stream st
+template <typename T>
+static int ckpt_copy(stfwi_plan *p, int k, int s0, int s1, bool restore, hipStream_t st)
+{
+    const size_t es = sizeof(T);
+    char *slot = (char *)p->ckpt + (size_t)k * (p->ck_state + p->ck_psix + p->ck_psiz) * es;
+    struct R { char *buf; char *ck; i64 per_shot; } r[3] = {
+        {(char *)p->state, slot, 5 * p->fstride},
+        {(char *)p->psix_f, slot + p->ck_state * es, 5 * p->psix_len},
+        {(char *)p->psiz_f, slot + (p->ck_state + p->ck_psix) * es, 5 * p->psiz_len}};
+    for (auto &x : r) {
+        const size_t off = (size_t)s0 * x.per_shot * es, n = (size_t)(s1 - s0) * x.per_shot * es;
+        if (restore) HIP_TRY(hipMemcpyAsync(x.buf + off, x.ck + off, n, hipMemcpyDeviceToDevice, st));
+        else HIP_TRY(hipMemcpyAsync(x.ck + off, x.buf + off, n, hipMemcpyDeviceToDevice, st));
+    }
+    return 0;
 }
 
 template <typename T, int FD>
@@ -473,13 +517,32 @@ static int forward_t(stfwi_plan *p, void *rec, int rec_mask, void *illum, int il
     const int rblk = rec ? (p->max_nrec + 255) / 256 : 0;
     int g0[STFWI_MAX_GROUPS + 1];
     const int G = group_bounds(p, g0);
+    // with checkpoints the history holds the last segment only: frame 0 is
+    // the carry-in of its first step
+    const int K = p->ck;
+    const int last0 = K ? ((p->nt_run - 1) / K) * K : 0;
+    if (K && p->hist) {
+        a.hist_t0 = last0 - 1;
+        fill<T>(p->hist, (i64)S * p->hist_nt * 3 * p->np, st);
+    }
     hipStream_t gs[STFWI_MAX_GROUPS];
     HIP_TRY(fork_groups(p, G, st, gs));
     for (int it = 0; it < p->nt_run; ++it)
         for (int k = 0; k < G; ++k) {
             a.shot0 = g0[k];
-            launch_fwd<T, FD>(0, p->tz[0], a, rblk, g0[k + 1] - g0[k], gs[k], it);
-            launch_fwd<T, FD>(1, p->tz[1], a, rblk, g0[k + 1] - g0[k], gs[k], it);
+            FwdArgs<T> b = a;
+            if (K && p->hist) {
+                if (it > 0 && it % K == 0) {
+                    const int rc = ckpt_copy<T>(p, it / K, g0[k], g0[k + 1], false, gs[k]);
+                    if (rc) return rc;
+                }
+                if (it < a.hist_t0) {   // history of earlier segments is recomputed later
+                    b.hist = nullptr;
+                    b.hist2_mode = 0;
+                }
+            }
+            launch_fwd<T, FD>(0, p->tz[0], b, rblk, g0[k + 1] - g0[k], gs[k], it);
+            launch_fwd<T, FD>(1, p->tz[1], b, rblk, g0[k + 1] - g0[k], gs[k], it);
         }
     // exx/ezz receivers of the last step
     if (rec)
@@ -519,7 +582,7 @@ static int adjoint_t(stfwi_plan *p, const void *rec_adj, void *src_adj, void *sn
 {
     AdjArgs<T> a = adj_args<T>(p, rec_adj, src_adj, snap, snap_mask);
     const int S = p->n_shots;
-    fill<T>(p->state, (i64)S * 5 * p->fstride, st);
+    fill<T>(a.state, (i64)S * 5 * p->fstride, st);
     for (int k = 0; k < 2; ++k) {
         fill<T>(p->psix_a[k], (i64)S * 4 * p->psix_len, st);
         fill<T>(p->psiz_a[k], (i64)S * 4 * p->psiz_len, st);
@@ -529,16 +592,50 @@ static int adjoint_t(stfwi_plan *p, const void *rec_adj, void *src_adj, void *sn
     const int G = group_bounds(p, g0);
     hipStream_t gs[STFWI_MAX_GROUPS];
     HIP_TRY(fork_groups(p, G, st, gs));
-    for (int it = p->nt_run - 1; it >= 0; --it) {
-        const int cur = it & 1, prev = cur ^ 1;
-        a.psix_o = (const T *)p->psix_a[prev]; a.psiz_o = (const T *)p->psiz_a[prev];
-        a.psix_n = (T *)p->psix_a[cur]; a.psiz_n = (T *)p->psiz_a[cur];
-        for (int k = 0; k < G; ++k) {
-            const int ns = g0[k + 1] - g0[k];
-            a.shot0 = g0[k];
-            hipLaunchKernelGGL(k_adj_inject<T>, dim3((p->max_nuc + 63) / 64, 1, ns), dim3(64), 0, gs[k], a, it);
-            launch_adj<T, FD>(2, p->tz[2], a, ns, gs[k], it);
-            launch_adj<T, FD>(3, p->tz[3], a, ns, gs[k], it);
+    // segments of the reverse sweep: one with the full history, or the
+    // checkpoint segments last to first, each but the last recomputed
+    const int K = (p->ck && a.imaging) ? p->ck : p->nt_run;
+    const int nseg = (p->nt_run + K - 1) / K;
+    FwdArgs<T> f = fwd_args<T>(p, nullptr, 0, nullptr, 0, nullptr, 0);
+    f.dxu0 = nullptr;                 // kept from the first forward
+    for (int seg = nseg - 1; seg >= 0; --seg) {
+        const int t_lo = seg * K, t_hi = std::min(t_lo + K, p->nt_run);
+        if (p->ck && a.imaging) {
+            a.hist_t0 = f.hist_t0 = t_lo - 1;
+            if (seg < nseg - 1) {
+                for (int k = 0; k < G; ++k) {
+                    const int s0 = g0[k], ns = g0[k + 1] - g0[k];
+                    if (seg == 0) {
+                        fill<T>((T *)p->state + (i64)s0 * 5 * p->fstride, (i64)ns * 5 * p->fstride, gs[k]);
+                        fill<T>((T *)p->psix_f + (i64)s0 * 5 * p->psix_len, (i64)ns * 5 * p->psix_len, gs[k]);
+                        fill<T>((T *)p->psiz_f + (i64)s0 * 5 * p->psiz_len, (i64)ns * 5 * p->psiz_len, gs[k]);
+                    } else {
+                        const int rc = ckpt_copy<T>(p, seg, s0, g0[k + 1], true, gs[k]);
+                        if (rc) return rc;
+                    }
+                    const int nb = (int)std::min<i64>((p->np + 255) / 256, 1024);
+                    hipLaunchKernelGGL(k_hist_from_state<T>, dim3(nb, ns), dim3(256), 0, gs[k], f.g,
+                                       (const T *)p->state, (i64)5 * p->fstride, (T *)p->hist, p->hist_nt, 0, s0);
+                }
+                for (int it = t_lo; it < t_hi; ++it)
+                    for (int k = 0; k < G; ++k) {
+                        f.shot0 = g0[k];
+                        launch_fwd<T, FD>(0, p->tz[0], f, 0, g0[k + 1] - g0[k], gs[k], it);
+                        launch_fwd<T, FD>(1, p->tz[1], f, 0, g0[k + 1] - g0[k], gs[k], it);
+                    }
+            }
+        }
+        for (int it = t_hi - 1; it >= t_lo; --it) {
+            const int cur = it & 1, prev = cur ^ 1;
+            a.psix_o = (const T *)p->psix_a[prev]; a.psiz_o = (const T *)p->psiz_a[prev];
+            a.psix_n = (T *)p->psix_a[cur]; a.psiz_n = (T *)p->psiz_a[cur];
+            for (int k = 0; k < G; ++k) {
+                const int ns = g0[k + 1] - g0[k];
+                a.shot0 = g0[k];
+                hipLaunchKernelGGL(k_adj_inject<T>, dim3((p->max_nuc + 63) / 64, 1, ns), dim3(64), 0, gs[k], a, it);
+                launch_adj<T, FD>(2, p->tz[2], a, ns, gs[k], it);
+                launch_adj<T, FD>(3, p->tz[3], a, ns, gs[k], it);
+            }
         }
     }
     HIP_TRY(hipGetLastError());
@@ -643,11 +740,13 @@ static int time_kernel_t(stfwi_plan *p, int which, int tz, int reps, hipStream_t
     HIP_TRY(hipEventCreate(&e1));
     if (which <= 1) {
         FwdArgs<T> a = fwd_args<T>(p, nullptr, 0, nullptr, 0, nullptr, 0);
+        if (p->ck) a.hist_t0 = it - 1;    // frames 0..2 of a checkpoint segment
         HIP_TRY(hipEventRecord(e0, st));
         for (int r = 0; r < reps; ++r) launch_fwd<T, FD>(which, tz, a, 0, p->n_shots, st, it);
         HIP_TRY(hipEventRecord(e1, st));
     } else {
         AdjArgs<T> a = adj_args<T>(p, nullptr, nullptr, nullptr, 0);
+        if (p->ck) a.hist_t0 = it - 1;
         a.psix_o = (const T *)p->psix_a[0]; a.psiz_o = (const T *)p->psiz_a[0];
         a.psix_n = (T *)p->psix_a[1]; a.psiz_n = (T *)p->psiz_a[1];
         HIP_TRY(hipEventRecord(e0, st));

@@ -44,12 +44,14 @@ class Plan:
     batch sizing: n_shots_max, nt_max, nrec_max; ``imaging`` ('none',
     'zhou' = backend 'adjoint', 'exact' = backend 'ad'); ``grad`` ('vs',
     'vs_vp', 'all' = the reference's ``invert``); ``store_history`` keeps the
-    strain history for the imaging adjoint.
+    strain history for the imaging adjoint; ``ckpt_every`` = K > 0 keeps
+    only forward-state checkpoints every K steps plus one K-step segment of
+    history and recomputes each segment during the adjoint (same results).
     """
 
     def __init__(self, nz_dom, nx_dom, dx, dz, dt, fc, pad, fd_order=2, free_surface=True,
                  dtype=torch.float32, n_shots_max=1, nt_max=1, nrec_max=1, imaging="none",
-                 grad="vs", store_history=False, device=None):
+                 grad="vs", store_history=False, device=None, ckpt_every=0):
         if fd_order not in (2, 4, 8):
             raise NotImplementedError(f"fd_order={fd_order} not supported (use 2, 4, or 8).")
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
@@ -63,11 +65,12 @@ class Plan:
         self.nt_max, self.nrec_max, self.n_shots_max = int(nt_max), int(nrec_max), int(n_shots_max)
         self.imaging = imaging
         self.grad = grad
+        self.ckpt_every = int(ckpt_every)
         self.shape = padded_shape(self.nz_dom, self.nx_dom, self.pad, self.free_surface)
         cfg = _abi.Config(self.nz_dom, self.nx_dom, self.pad, int(self.free_surface), self.fd_order,
                           self.dx, self.dz, self.dt, self.fc, _code(dtype), self.n_shots_max,
                           self.nt_max, self.nrec_max, _abi.IMAGING[imaging], _abi.GRAD[grad],
-                          int(bool(store_history)))
+                          int(bool(store_history)), int(ckpt_every))
         h = C.c_void_p()
         with torch.cuda.device(self.device):
             check(lib().stfwi_plan_create(C.byref(cfg), C.byref(h)))

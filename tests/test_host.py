@@ -44,7 +44,8 @@ def test_library_exports_every_declared_symbol(lib):
 def _cfg(**kw):
     from stfwi_amd import _abi
     base = dict(nz_dom=20, nx_dom=30, pad=6, free_surface=1, fd_order=8, dx=10.0, dz=10.0, dt=1e-3, fc=10.0,
-                dtype=_abi.F32, n_shots_max=1, nt_max=10, nrec_max=3, imaging=0, grad_params=1, store_history=0)
+                dtype=_abi.F32, n_shots_max=1, nt_max=10, nrec_max=3, imaging=0, grad_params=1, store_history=0,
+                ckpt_every=0)
     base.update(kw)
     return _abi.Config(**base)
 
@@ -52,7 +53,8 @@ def _cfg(**kw):
 @pytest.mark.parametrize("bad,code", [
     (dict(fd_order=6), -2), (dict(fd_order=3), -2), (dict(dtype=7), -1), (dict(nt_max=0), -1),
     (dict(n_shots_max=0), -1), (dict(imaging=5), -1), (dict(grad_params=0), -1), (dict(dt=0.0), -1),
-    (dict(nz_dom=1, nx_dom=1, pad=1), -1),
+    (dict(nz_dom=1, nx_dom=1, pad=1), -1), (dict(ckpt_every=-1), -1),
+    (dict(ckpt_every=4, imaging=1, grad_params=3, store_history=1), -2),
 ])
 def test_plan_create_rejects_bad_config_without_device(lib, bad, code):
     h = C.c_void_p()
