@@ -206,6 +206,15 @@ int stfwi_autotune(stfwi_plan *plan, int reps, void *stream, int *tz_out, double
  * forward strain, adjoint strain, adjoint velocity). */
 int stfwi_set_tiles(stfwi_plan *plan, const int *tz);
 
+/* Graph replay of the time loops (default off; no reference counterpart):
+ * stfwi_forward without illumination or snapshots and stfwi_adjoint without
+ * snapshots capture their whole time loop into a hipGraph the first time a
+ * launch configuration (shots, record length, tiles, groups, channels) is
+ * seen and replay it afterwards, with receivers, receiver adjoints and the
+ * source adjoint staged through plan-owned buffers.  Results are identical
+ * to the eager launches; enable = 0 (the default) launches eagerly. */
+int stfwi_set_graphs(stfwi_plan *plan, int enable);
+
 /* Split the shots into `groups` (1..4) contiguous groups whose time loops
  * run concurrently on the plan's own streams (default priority), joined
  * back into the caller's stream; results do not depend on the grouping. */

@@ -72,6 +72,16 @@ struct stfwi_plan {
     int groups = 1;
     hipStream_t gstream[STFWI_MAX_GROUPS] = {};
     hipEvent_t ev_fork = nullptr, ev_join[STFWI_MAX_GROUPS] = {};
+    // graph replay of the time loops: the forward and the adjoint are each
+    // captured once per launch configuration (key) and replayed; their
+    // inputs and outputs go through plan-owned buffers so that the captured
+    // kernel arguments stay valid
+    bool graphs = false;
+    hipStream_t cap = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    std::vector<long long> gkey[2];
+    void *rec_int = nullptr, *radj_int = nullptr, *sadj_int = nullptr, *wav_int = nullptr;
     ShotDev *shots_d = nullptr;
     i64 *rec_cell_d = nullptr, *uc_cell_d = nullptr;
     int *uc_ptr_d = nullptr, *uc_rec_d = nullptr;
@@ -136,12 +146,21 @@ extern "C" long long stfwi_plan_bytes(const stfwi_plan *plan) { return plan ? pl
 extern "C" int stfwi_plan_destroy(stfwi_plan *plan)
 {
     if (!plan) return 0;
+    // work of the plan may still be queued: finish it before freeing
+    if (plan->cap) (void)hipStreamSynchronize(plan->cap);
+    for (int k = 0; k < STFWI_MAX_GROUPS; ++k)
+        if (plan->gstream[k]) (void)hipStreamSynchronize(plan->gstream[k]);
     for (void *a : plan->allocs) (void)hipFree(a);
     for (int k = 0; k < STFWI_MAX_GROUPS; ++k) {
         if (plan->gstream[k]) (void)hipStreamDestroy(plan->gstream[k]);
         if (plan->ev_join[k]) (void)hipEventDestroy(plan->ev_join[k]);
     }
     if (plan->ev_fork) (void)hipEventDestroy(plan->ev_fork);
+    for (int k = 0; k < 2; ++k)
+        if (plan->gexec[k]) (void)hipGraphExecDestroy(plan->gexec[k]);
+    if (plan->cap) (void)hipStreamDestroy(plan->cap);
+    if (plan->ev_in) (void)hipEventDestroy(plan->ev_in);
+    if (plan->ev_out) (void)hipEventDestroy(plan->ev_out);
     delete plan;
     return 0;
 }
@@ -216,6 +235,10 @@ extern "C" int stfwi_plan_create(const stfwi_config *cfg, stfwi_plan **out)
         rc |= dalloc(p, &p->gacc, (size_t)S * p->nacc * p->np * es);
     }
     rc |= dalloc(p, (void **)&p->shots_d, S * sizeof(ShotDev));
+    rc |= dalloc(p, &p->rec_int, (size_t)S * 4 * c.nt_max * std::max(c.nrec_max, 1) * es);
+    rc |= dalloc(p, &p->radj_int, (size_t)S * 4 * c.nt_max * std::max(c.nrec_max, 1) * es);
+    rc |= dalloc(p, &p->sadj_int, (size_t)S * c.nt_max * es);
+    rc |= dalloc(p, &p->wav_int, (size_t)S * c.nt_max * es);
     rc |= dalloc(p, (void **)&p->rec_cell_d, (S * std::max(c.nrec_max, 1)) * sizeof(i64));
     rc |= dalloc(p, (void **)&p->uc_cell_d, (S * std::max(c.nrec_max, 1)) * sizeof(i64));
     rc |= dalloc(p, (void **)&p->uc_ptr_d, (S * std::max(c.nrec_max, 1) + 1) * sizeof(int));
@@ -232,6 +255,9 @@ extern "C" int stfwi_plan_create(const stfwi_config *cfg, stfwi_plan **out)
         HIP_TRY(hipEventCreateWithFlags(&p->ev_join[k], hipEventDisableTiming));
     }
     HIP_TRY(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
+    HIP_TRY(hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&p->ev_in, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&p->ev_out, hipEventDisableTiming));
     // margins of the moduli buffers must be zero
     hipStream_t st = nullptr;
     if (c.dtype == STFWI_F64) fill<double>(p->mod, 4 * p->fstride + p->nx, st);
@@ -326,13 +352,15 @@ extern "C" int stfwi_set_shots(stfwi_plan *p, const stfwi_shot *shots, int n_sho
     HIP_TRY(hipMemcpyAsync(p->uc_cell_d, ucell.data(), ucell.size() * sizeof(i64), hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(p->uc_ptr_d, uptr.data(), uptr.size() * sizeof(int), hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(p->uc_rec_d, urec.data(), urec.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    // the wavelets are read by the kernels of every later time loop: keep a copy
+    HIP_TRY(hipMemcpyAsync(p->wav_int, wavelets, (size_t)n_shots * c.nt_max * p->esz, hipMemcpyDeviceToDevice, st));
     // the host vectors die at return: wait for the copies
     HIP_TRY(hipStreamSynchronize(st));
     p->n_shots = n_shots;
     p->nt_run = nt_run;
     p->max_nuc = max_nuc;
     p->max_nrec = max_nrec;
-    p->wav = wavelets;
+    p->wav = p->wav_int;
     p->fwd_done = false;
     return 0;
 }
@@ -555,6 +583,67 @@ static int forward_t(stfwi_plan *p, void *rec, int rec_mask, void *illum, int il
     return 0;
 }
 
+// zero `len` elements at ptr + k * pitch for k < rows (element size es)
+static void fill_rows(void *ptr, i64 rows, i64 pitch, i64 len, int es, hipStream_t st)
+{
+    for (i64 k = 0; k < rows; ++k) {
+        if (es == 8) fill<double>((double *)ptr + k * pitch, len, st);
+        else fill<float>((float *)ptr + k * pitch, len, st);
+    }
+}
+
+// Replay (capturing first when the key changed) of the work `body` enqueues
+// on a stream; the replay is ordered after the caller's stream st and st
+// after it.
+template <typename F>
+static int run_graph(stfwi_plan *p, int which, const std::vector<long long> &key, hipStream_t st, F body)
+{
+    if (!p->gexec[which] || p->gkey[which] != key) {
+        if (p->gexec[which]) {
+            // the previous executable may still be running on the capture
+            // stream: let it finish before it is destroyed
+            HIP_TRY(hipStreamSynchronize(p->cap));
+            (void)hipGraphExecDestroy(p->gexec[which]);
+            p->gexec[which] = nullptr;
+        }
+        HIP_TRY(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
+        const int rc = body(p->cap);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(p->cap, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(STFWI_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+        const hipError_t e2 = hipGraphInstantiate(&p->gexec[which], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e2 != hipSuccess) {
+            p->gexec[which] = nullptr;
+            return fail(STFWI_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e2));
+        }
+        p->gkey[which] = key;
+    }
+    HIP_TRY(hipEventRecord(p->ev_in, st));
+    HIP_TRY(hipStreamWaitEvent(p->cap, p->ev_in, 0));
+    HIP_TRY(hipGraphLaunch(p->gexec[which], p->cap));
+    HIP_TRY(hipEventRecord(p->ev_out, p->cap));
+    HIP_TRY(hipStreamWaitEvent(st, p->ev_out, 0));
+    return 0;
+}
+
+static int forward_dispatch(stfwi_plan *p, void *rec, int rec_mask, void *illum, int illum_comp, void *snap,
+                            int snap_mask, hipStream_t st)
+{
+    const int fd = p->cfg.fd_order;
+    if (p->cfg.dtype == STFWI_F64)
+        return fd == 8 ? forward_t<double, 8>(p, rec, rec_mask, illum, illum_comp, snap, snap_mask, st)
+             : fd == 4 ? forward_t<double, 4>(p, rec, rec_mask, illum, illum_comp, snap, snap_mask, st)
+                       : forward_t<double, 2>(p, rec, rec_mask, illum, illum_comp, snap, snap_mask, st);
+    return fd == 8 ? forward_t<float, 8>(p, rec, rec_mask, illum, illum_comp, snap, snap_mask, st)
+         : fd == 4 ? forward_t<float, 4>(p, rec, rec_mask, illum, illum_comp, snap, snap_mask, st)
+                   : forward_t<float, 2>(p, rec, rec_mask, illum, illum_comp, snap, snap_mask, st);
+}
+
 extern "C" int stfwi_forward(stfwi_plan *p, void *rec, int rec_mask, void *illum, int illum_comp, void *snap,
                              int snap_mask, void *stream)
 {
@@ -564,15 +653,27 @@ extern "C" int stfwi_forward(stfwi_plan *p, void *rec, int rec_mask, void *illum
     if (illum && (illum_comp < 0 || illum_comp > 3)) return fail(STFWI_ERR_INVALID, "illum_comp=%d", illum_comp);
     hipStream_t st = (hipStream_t)stream;
     int rc;
-    const int fd = p->cfg.fd_order;
-    if (p->cfg.dtype == STFWI_F64)
-        rc = fd == 8 ? forward_t<double, 8>(p, rec, rec_mask, illum, illum_comp, snap, snap_mask, st)
-           : fd == 4 ? forward_t<double, 4>(p, rec, rec_mask, illum, illum_comp, snap, snap_mask, st)
-                     : forward_t<double, 2>(p, rec, rec_mask, illum, illum_comp, snap, snap_mask, st);
-    else
-        rc = fd == 8 ? forward_t<float, 8>(p, rec, rec_mask, illum, illum_comp, snap, snap_mask, st)
-           : fd == 4 ? forward_t<float, 4>(p, rec, rec_mask, illum, illum_comp, snap, snap_mask, st)
-                     : forward_t<float, 2>(p, rec, rec_mask, illum, illum_comp, snap, snap_mask, st);
+    if (p->graphs && !illum && !snap) {
+        // receivers go through rec_int: the requested channels are zeroed in
+        // the graph and copied out after it
+        const int mask = rec ? (rec_mask & 15) : 0;
+        const i64 blk = (i64)p->cfg.nt_max * std::max(p->cfg.nrec_max, 1);
+        const std::vector<long long> key = {p->n_shots, p->nt_run, p->tz[0], p->tz[1], p->groups, mask, p->max_nrec};
+        rc = run_graph(p, 0, key, st, [&](hipStream_t cs) {
+            for (int ch = 0; ch < 4; ++ch)
+                if (mask & (1 << ch))
+                    fill_rows((char *)p->rec_int + ch * blk * p->esz, p->n_shots, 4 * blk, blk, p->esz, cs);
+            return forward_dispatch(p, mask ? p->rec_int : nullptr, mask, nullptr, 0, nullptr, 0, cs);
+        });
+        for (int ch = 0; rc == 0 && ch < 4; ++ch)
+            if (mask & (1 << ch)) {
+                const size_t row = (size_t)blk * p->esz, pitch = 4 * row;
+                HIP_TRY(hipMemcpy2DAsync((char *)rec + ch * row, pitch, (char *)p->rec_int + ch * row, pitch, row,
+                                         p->n_shots, hipMemcpyDeviceToDevice, st));
+            }
+    } else {
+        rc = forward_dispatch(p, rec, rec_mask, illum, illum_comp, snap, snap_mask, st);
+    }
     p->fwd_done = rc == 0 && p->hist != nullptr;
     return rc;
 }
@@ -652,15 +753,31 @@ extern "C" int stfwi_adjoint(stfwi_plan *p, const void *rec_adj, void *src_adj, 
         return fail(STFWI_ERR_STATE, "imaging needs a stfwi_forward with the same shots first");
     hipStream_t st = (hipStream_t)stream;
     const int fd = p->cfg.fd_order;
+    auto dispatch = [&](const void *ra, void *sa, void *sn, int sm, hipStream_t s2) {
+        if (p->cfg.dtype == STFWI_F64)
+            return fd == 8 ? adjoint_t<double, 8>(p, ra, sa, sn, sm, s2)
+                 : fd == 4 ? adjoint_t<double, 4>(p, ra, sa, sn, sm, s2)
+                           : adjoint_t<double, 2>(p, ra, sa, sn, sm, s2);
+        return fd == 8 ? adjoint_t<float, 8>(p, ra, sa, sn, sm, s2)
+             : fd == 4 ? adjoint_t<float, 4>(p, ra, sa, sn, sm, s2)
+                       : adjoint_t<float, 2>(p, ra, sa, sn, sm, s2);
+    };
     int rc;
-    if (p->cfg.dtype == STFWI_F64)
-        rc = fd == 8 ? adjoint_t<double, 8>(p, rec_adj, src_adj, snap, snap_mask, st)
-           : fd == 4 ? adjoint_t<double, 4>(p, rec_adj, src_adj, snap, snap_mask, st)
-                     : adjoint_t<double, 2>(p, rec_adj, src_adj, snap, snap_mask, st);
-    else
-        rc = fd == 8 ? adjoint_t<float, 8>(p, rec_adj, src_adj, snap, snap_mask, st)
-           : fd == 4 ? adjoint_t<float, 4>(p, rec_adj, src_adj, snap, snap_mask, st)
-                     : adjoint_t<float, 2>(p, rec_adj, src_adj, snap, snap_mask, st);
+    if (p->graphs && !snap) {
+        const size_t rbytes = (size_t)p->n_shots * 4 * p->cfg.nt_max * std::max(p->cfg.nrec_max, 1) * p->esz;
+        HIP_TRY(hipMemcpyAsync(p->radj_int, rec_adj, rbytes, hipMemcpyDeviceToDevice, st));
+        const int want = src_adj ? 1 : 0;
+        const std::vector<long long> key = {p->n_shots, p->nt_run, p->tz[2], p->tz[3], p->groups, want, p->max_nuc};
+        rc = run_graph(p, 1, key, st, [&](hipStream_t cs) {
+            if (want) fill_rows(p->sadj_int, 1, 0, (i64)p->n_shots * p->cfg.nt_max, p->esz, cs);
+            return dispatch(p->radj_int, want ? p->sadj_int : nullptr, nullptr, 0, cs);
+        });
+        if (rc == 0 && want)
+            HIP_TRY(hipMemcpyAsync(src_adj, p->sadj_int, (size_t)p->n_shots * p->cfg.nt_max * p->esz,
+                                   hipMemcpyDeviceToDevice, st));
+    } else {
+        rc = dispatch(rec_adj, src_adj, snap, snap_mask, st);
+    }
     p->adj_done = rc == 0 && p->gacc != nullptr && p->cfg.imaging != 0;
     return rc;
 }
@@ -1019,5 +1136,12 @@ extern "C" int stfwi_bound_mask(const double *x, const double *g, const double *
     const long long blocks = std::min<long long>((n + 255) / 256, 4096);
     hipLaunchKernelGGL(k_bound_mask, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, x, g, lo, hi, n, v);
     HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int stfwi_set_graphs(stfwi_plan *p, int enable)
+{
+    if (!p) return fail(STFWI_ERR_INVALID, "null plan");
+    p->graphs = enable != 0;
     return 0;
 }

@@ -228,6 +228,11 @@ class Plan:
                  for i, n in enumerate(names)}
         return list(tz), table
 
+    def set_graphs(self, enable=True):
+        """Replay the forward / adjoint time loops as captured hipGraphs, or
+        launch them eagerly (the default)."""
+        check(lib().stfwi_set_graphs(self._h, int(bool(enable))))
+
     def set_groups(self, groups):
         """Run the shots as `groups` concurrent stream groups (1..4)."""
         check(lib().stfwi_set_groups(self._h, int(groups)))
