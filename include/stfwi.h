@@ -252,6 +252,45 @@ int stfwi_step_clip(const double *x, const double *d, double t, const double *lo
 int stfwi_bound_mask(const double *x, const double *g, const double *lo, const double *hi, long long n, double *v,
                      void *stream);
 
+/* ---- Learned gradient preconditioner (no reference counterpart; the
+ * reference's plain method has no learned part).  A small U-Net whose
+ * convolutions run on the matrix cores: operands rounded to BF16, FP32
+ * accumulation.  Activations are FP32 NCHW (batch x channels x h x w),
+ * weights FP32 [cout][cin][k][k], k = 1 or 3, stride 1, zero 'same'
+ * padding.  All pointers are device pointers. ---- */
+
+/* out = conv(in, weight) + bias, then max(., 0) if relu. */
+int stfwi_conv2d_bf16(const float *in, int batch, int cin, int h, int w, const float *weight, const float *bias,
+                      int cout, int ksize, int relu, float *out, void *stream);
+/* din = gradient of the convolution's input from dout (transposed,
+ * flipped weights). */
+int stfwi_conv2d_bf16_dgrad(const float *dout, int batch, int cout, int h, int w, const float *weight, int cin,
+                            int ksize, float *din, void *stream);
+/* dw (and db if not NULL) = gradient of the weights (bias) from in and dout,
+ * summed over the batch; accumulate = 1 adds to dw / db.  work is a scratch
+ * buffer of stfwi_conv2d_bf16_wgrad_work(cin, cout, ksize) floats. */
+long long stfwi_conv2d_bf16_wgrad_work(int cin, int cout, int ksize);
+int stfwi_conv2d_bf16_wgrad(const float *in, const float *dout, int batch, int cin, int cout, int h, int w, int ksize,
+                            float *dw, float *db, int accumulate, float *work, long long work_elems, void *stream);
+/* g *= (y > 0) */
+int stfwi_relu_bwd(float *g, const float *y, long long n, void *stream);
+/* 2 x 2 average pooling of `planes` h x w planes (h, w even) or, with
+ * backward = 1, its gradient (in: h/2 x w/2 gradients, out: h x w). */
+int stfwi_pool2(const float *in, long long planes, int h, int w, float *out, int backward, void *stream);
+/* Nearest x2 upsampling of h x w planes into 2h x 2w planes laid out with
+ * big_plane_stride (so it can fill part of a concatenation buffer), or with
+ * backward = 1 its gradient (in: the 2h x 2w gradients, out: h x w). */
+int stfwi_up2(const float *in, int batch, int planes, int h, int w, float *out, long long big_plane_stride,
+              long long small_batch_stride, long long big_batch_stride, int backward, void *stream);
+/* Adam step (Kingma & Ba, Alg. 1) of step number `step` >= 1 on FP32 p. */
+int stfwi_adam(float *p, const float *g, float *m, float *v, long long n, double lr, double b1, double b2, double eps,
+               int step, void *stream);
+/* Weighted MSE: g = 2 w (y - t) inv_wsum, *loss = sum w (y - t)^2 (device
+ * scalar; multiply by inv_wsum for the mean); part is 256 floats of
+ * scratch; wgt may be NULL (weight 1). */
+int stfwi_mse(const float *y, const float *t, const float *wgt, long long n, double inv_wsum, float *g, float *part,
+              float *loss, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,20 @@ SIGNATURES = {
     "stfwi_dot64": (C.c_int, [_VP, _VP, C.c_longlong, _VP, _VP]),
     "stfwi_step_clip": (C.c_int, [_VP, _VP, C.c_double, _VP, _VP, C.c_longlong, _VP, _VP]),
     "stfwi_bound_mask": (C.c_int, [_VP, _VP, _VP, _VP, C.c_longlong, _VP, _VP]),
+    "stfwi_conv2d_bf16": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, _VP, _VP, C.c_int, C.c_int,
+                                    C.c_int, _VP, _VP]),
+    "stfwi_conv2d_bf16_dgrad": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, _VP, C.c_int, C.c_int, _VP,
+                                          _VP]),
+    "stfwi_conv2d_bf16_wgrad_work": (C.c_longlong, [C.c_int, C.c_int, C.c_int]),
+    "stfwi_conv2d_bf16_wgrad": (C.c_int, [_VP, _VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _VP, _VP,
+                                          C.c_int, _VP, C.c_longlong, _VP]),
+    "stfwi_relu_bwd": (C.c_int, [_VP, _VP, C.c_longlong, _VP]),
+    "stfwi_pool2": (C.c_int, [_VP, C.c_longlong, C.c_int, C.c_int, _VP, C.c_int, _VP]),
+    "stfwi_up2": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, _VP, C.c_longlong, C.c_longlong, C.c_longlong,
+                            C.c_int, _VP]),
+    "stfwi_adam": (C.c_int, [_VP, _VP, _VP, _VP, C.c_longlong, C.c_double, C.c_double, C.c_double, C.c_double,
+                             C.c_int, _VP]),
+    "stfwi_mse": (C.c_int, [_VP, _VP, _VP, C.c_longlong, C.c_double, _VP, _VP, _VP, _VP]),
 }
 
 

@@ -16,6 +16,7 @@
 #include "fwi_kernels.h"
 #include "post_kernels.h"
 #include "lbfgs_kernels.h"
+#include "net_kernels.h"
 
 using namespace stfwi;
 
@@ -1143,5 +1144,145 @@ extern "C" int stfwi_set_graphs(stfwi_plan *p, int enable)
 {
     if (!p) return fail(STFWI_ERR_INVALID, "null plan");
     p->graphs = enable != 0;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// learned preconditioner: BF16-MFMA convolutions and FP32 layer kernels
+// ---------------------------------------------------------------------------
+static int grid1(long long n) { return (int)std::min<long long>((n + 255) / 256, 8192); }
+
+extern "C" int stfwi_conv2d_bf16(const float *in, int batch, int cin, int h, int w, const float *weight,
+                                 const float *bias, int cout, int ksize, int relu, float *out, void *stream)
+{
+    if (!in || !weight || !out || batch < 1 || cin < 1 || cout < 1 || h < 1 || w < 1)
+        return fail(STFWI_ERR_INVALID, "bad argument");
+    if (ksize != 1 && ksize != 3) return fail(STFWI_ERR_UNSUPPORTED, "ksize=%d (use 1 or 3)", ksize);
+    const long long P = (long long)h * w;
+    if (P > (1LL << 30)) return fail(STFWI_ERR_INVALID, "image too large");
+    dim3 grid((unsigned)((P + 255) / 256), (unsigned)((cout + 15) / 16), (unsigned)batch);
+    if (ksize == 3)
+        hipLaunchKernelGGL((k_conv_mfma<3, false>), grid, dim3(256), 0, (hipStream_t)stream, in, cin, h, w, weight,
+                           bias, cout, out, relu);
+    else
+        hipLaunchKernelGGL((k_conv_mfma<1, false>), grid, dim3(256), 0, (hipStream_t)stream, in, cin, h, w, weight,
+                           bias, cout, out, relu);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int stfwi_conv2d_bf16_dgrad(const float *dout, int batch, int cout, int h, int w, const float *weight,
+                                       int cin, int ksize, float *din, void *stream)
+{
+    if (!dout || !weight || !din || batch < 1 || cin < 1 || cout < 1 || h < 1 || w < 1)
+        return fail(STFWI_ERR_INVALID, "bad argument");
+    if (ksize != 1 && ksize != 3) return fail(STFWI_ERR_UNSUPPORTED, "ksize=%d (use 1 or 3)", ksize);
+    const long long P = (long long)h * w;
+    dim3 grid((unsigned)((P + 255) / 256), (unsigned)((cin + 15) / 16), (unsigned)batch);
+    // a convolution of dout (cout channels) into cin channels with the
+    // transposed, flipped weights
+    if (ksize == 3)
+        hipLaunchKernelGGL((k_conv_mfma<3, true>), grid, dim3(256), 0, (hipStream_t)stream, dout, cout, h, w, weight,
+                           nullptr, cin, din, 0);
+    else
+        hipLaunchKernelGGL((k_conv_mfma<1, true>), grid, dim3(256), 0, (hipStream_t)stream, dout, cout, h, w, weight,
+                           nullptr, cin, din, 0);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" long long stfwi_conv2d_bf16_wgrad_work(int cin, int cout, int ksize)
+{
+    return 256LL * cout * cin * ksize * ksize;
+}
+
+extern "C" int stfwi_conv2d_bf16_wgrad(const float *in, const float *dout, int batch, int cin, int cout, int h, int w,
+                                       int ksize, float *dw, float *db, int accumulate, float *work,
+                                       long long work_elems, void *stream)
+{
+    if (!in || !dout || !dw || !work || batch < 1 || cin < 1 || cout < 1 || h < 1 || w < 1)
+        return fail(STFWI_ERR_INVALID, "bad argument");
+    if (ksize != 1 && ksize != 3) return fail(STFWI_ERR_UNSUPPORTED, "ksize=%d (use 1 or 3)", ksize);
+    const long long K = (long long)cin * ksize * ksize, PT = (long long)batch * h * w;
+    if (work_elems < stfwi_conv2d_bf16_wgrad_work(cin, cout, ksize))
+        return fail(STFWI_ERR_INVALID, "work buffer holds %lld elements, needs %lld", work_elems,
+                    stfwi_conv2d_bf16_wgrad_work(cin, cout, ksize));
+    // up to 256 pixel chunks of a multiple of 32 pixels
+    const long long pch = 32 * ((PT + 32 * 256 - 1) / (32 * 256));
+    const int nz = (int)((PT + pch - 1) / pch);
+    hipStream_t st = (hipStream_t)stream;
+    dim3 grid((unsigned)((K + 15) / 16), (unsigned)((cout + 15) / 16), (unsigned)nz);
+    if (ksize == 3)
+        hipLaunchKernelGGL(k_conv_wgrad_mfma<3>, grid, dim3(64), 0, st, in, cin, h, w, dout, cout, batch, (int)pch,
+                           work);
+    else
+        hipLaunchKernelGGL(k_conv_wgrad_mfma<1>, grid, dim3(64), 0, st, in, cin, h, w, dout, cout, batch, (int)pch,
+                           work);
+    const long long n = (long long)cout * K;
+    hipLaunchKernelGGL(k_sum_chunks, dim3(grid1(n)), dim3(256), 0, st, work, nz, n, dw, accumulate);
+    if (db) {
+        // bias gradient: per-channel sums into work, then into db
+        hipLaunchKernelGGL(k_bias_grad, dim3(cout), dim3(256), 0, st, dout, cout, batch, h * w, work);
+        hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(256), 0, st, work, 1, (long long)cout, db, accumulate);
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int stfwi_relu_bwd(float *g, const float *y, long long n, void *stream)
+{
+    if (!g || !y || n < 1) return fail(STFWI_ERR_INVALID, "bad argument");
+    hipLaunchKernelGGL(k_relu_bwd, dim3(grid1(n)), dim3(256), 0, (hipStream_t)stream, g, y, n);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int stfwi_pool2(const float *in, long long planes, int h, int w, float *out, int backward, void *stream)
+{
+    if (!in || !out || planes < 1 || h < 2 || w < 2 || (h & 1) || (w & 1))
+        return fail(STFWI_ERR_INVALID, "bad argument (planes of even size needed)");
+    const long long n = backward ? planes * h * w : planes * (h / 2) * (w / 2);
+    if (backward)
+        hipLaunchKernelGGL(k_pool2_bwd, dim3(grid1(n)), dim3(256), 0, (hipStream_t)stream, in, planes, h, w, out);
+    else
+        hipLaunchKernelGGL(k_pool2, dim3(grid1(n)), dim3(256), 0, (hipStream_t)stream, in, planes, h, w, out);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int stfwi_up2(const float *in, int batch, int planes, int h, int w, float *out, long long big_plane_stride,
+                         long long small_batch_stride, long long big_batch_stride, int backward, void *stream)
+{
+    if (!in || !out || batch < 1 || planes < 1 || h < 1 || w < 1) return fail(STFWI_ERR_INVALID, "bad argument");
+    const long long n = (long long)batch * planes * h * w * (backward ? 1 : 4);
+    if (backward)   // in: gradient of the 2h x 2w planes, out: gradient of the h x w planes
+        hipLaunchKernelGGL(k_up2_bwd, dim3(grid1(n)), dim3(256), 0, (hipStream_t)stream, in, planes, h, w, out,
+                           big_plane_stride, batch, small_batch_stride, big_batch_stride);
+    else            // in: h x w planes, out: 2h x 2w planes
+        hipLaunchKernelGGL(k_up2, dim3(grid1(n)), dim3(256), 0, (hipStream_t)stream, in, planes, h, w, out,
+                           big_plane_stride, batch, small_batch_stride, big_batch_stride);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int stfwi_adam(float *p, const float *g, float *m, float *v, long long n, double lr, double b1, double b2,
+                          double eps, int step, void *stream)
+{
+    if (!p || !g || !m || !v || n < 1 || step < 1) return fail(STFWI_ERR_INVALID, "bad argument");
+    const float c1 = (float)(1.0 - std::pow(b1, step)), c2 = (float)(1.0 - std::pow(b2, step));
+    hipLaunchKernelGGL(k_adam, dim3(grid1(n)), dim3(256), 0, (hipStream_t)stream, p, g, m, v, n, (float)lr, (float)b1,
+                       (float)b2, (float)eps, c1, c2);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int stfwi_mse(const float *y, const float *t, const float *wgt, long long n, double inv_wsum, float *g,
+                         float *part, float *loss, void *stream)
+{
+    if (!y || !t || !g || !part || !loss || n < 1) return fail(STFWI_ERR_INVALID, "bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_mse, dim3(256), dim3(256), 0, st, y, t, wgt, n, (float)inv_wsum, g, part);
+    hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(1), 0, st, part, 256, 1LL, loss, 0);
+    HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -21,6 +21,14 @@ The optimiser is this package's float64 device L-BFGS (stfwi_amd.lbfgs:
 history 10, 20 line-search evaluations, gtol/ftol disabled as in the
 reference), not scipy's L-BFGS-B; bounds are handled by projecting the
 trial points.  Every gradient is one batched GPU evaluation of all shots.
+
+Two keyword arguments beyond the reference's (Vs-only stages):
+  learned  a stfwi_amd.learned.LearnedPreconditioner: its direction from the
+           raw Vs gradient and the stage's illumination is tried before every
+           L-BFGS step, which is taken instead whenever the learned direction
+           does not decrease the misfit (history: n_learned, n_fallback);
+  collect  a LearnedPreconditioner that receives (gradient, illumination,
+           accepted update) training pairs from the run.
 """
 from __future__ import annotations
 
@@ -83,7 +91,7 @@ def run_multiscale_fwi(*, Vs_init, Vp_init, rho_init, src_x, src_z, rec_x, rec_z
                        auto_scale: Optional[float] = None, K_margin: float = 0.04, cfl_coeff: Optional[float] = None,
                        vs_bounds=None, vp_bounds=None, rho_bounds=None, plateau_window: Optional[int] = None,
                        plateau_rel_tol: float = 1e-3, save_dir: Optional[str] = None,
-                       save_every: Optional[int] = None, verbose: bool = True):
+                       save_every: Optional[int] = None, verbose: bool = True, learned=None, collect=None):
     if not stages:
         raise ValueError("`stages` must be a non-empty list")
     from scipy.signal import butter
@@ -210,10 +218,25 @@ def run_multiscale_fwi(*, Vs_init, Vp_init, rho_init, src_x, src_z, rec_x, rec_z
                                 Vp=p_proj[1], rho=p_proj[2], iter=iter_count[0],
                                 loss=float(loss_history[-1]) if loss_history else float("nan"))
 
+        grid = (int(nz_dom), int(nx_dom))
+        vs_only = active == ["Vs"]
+
+        def learned_dir(x, g):
+            # the network sees the raw Vs gradient (the scaled one differs by
+            # a constant factor, which its input normalisation removes)
+            return learned.direction(g.reshape(grid), illumination).reshape(-1)
+
+        def on_step(x_old, g_old, x_new, g_new, was_learned):
+            collect.add_pair(g_old.reshape(grid), illumination, (x_new - x_old).reshape(grid))
+
         t0 = time.time()
+        n_learned = n_fallback = 0
         try:
-            res = minimize(f_and_g, x0, lo=lo_t, hi=hi_t, maxiter=n_iters, m=10, maxls=20, callback=iter_callback)
+            res = minimize(f_and_g, x0, lo=lo_t, hi=hi_t, maxiter=n_iters, m=10, maxls=20, callback=iter_callback,
+                           direction_fn=learned_dir if (learned is not None and vs_only) else None,
+                           on_step=on_step if (collect is not None and vs_only) else None)
             res_x, res_fun, ok = res["x"], float(res["fun"]), bool(res["success"])
+            n_learned, n_fallback = res["n_learned"], res["n_fallback"]
         except _PlateauStop:
             res_x = last_x[0]
             res_fun = loss_history[-1] if loss_history else float("nan")
@@ -223,7 +246,7 @@ def run_multiscale_fwi(*, Vs_init, Vp_init, rho_init, src_x, src_z, rec_x, rec_z
         history.append(dict(stage=istage, f_c=f_c, cutoff=cutoff, invert=invert, n_iters_max=n_iters,
                             n_evals=eval_count[0], loss_history=np.array(loss_history), final_loss=res_fun,
                             success=ok, time_s=t_stage, Vs=params[0].copy(), Vp=params[1].copy(),
-                            rho=params[2].copy()))
+                            rho=params[2].copy(), n_learned=n_learned, n_fallback=n_fallback))
         if verbose:
             print(f"  Stage {istage + 1} done in {t_stage:.1f}s  final_loss={res_fun:.4e}  success={ok}")
         if save_dir is not None:

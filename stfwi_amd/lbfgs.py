@@ -200,13 +200,23 @@ def _armijo_projected(ev: _Eval, f0, g0, x0, t0, c1, maxls):
 
 def minimize(fun: Callable, x0: torch.Tensor, lo: Optional[torch.Tensor] = None, hi: Optional[torch.Tensor] = None,
              maxiter: int = 50, m: int = 10, maxls: int = 20, c1: float = 1e-4, c2: float = 0.9,
-             callback: Optional[Callable] = None, memory: Optional[LBFGSMemory] = None):
+             callback: Optional[Callable] = None, memory: Optional[LBFGSMemory] = None,
+             direction_fn: Optional[Callable] = None, maxls_learned: int = 10, on_step: Optional[Callable] = None):
     """Minimise fun(x) -> (f, g) over float64 device vectors x, with optional
     per-entry bounds lo <= x <= hi (device tensors or None).
 
     One iteration = one accepted step; callback(x) runs after each.  Stops
     after maxiter iterations or when the line search finds no decrease.
-    Returns a dict: x, fun, jac, nit, nfev, success, message."""
+
+    direction_fn(x, g) -> d (or None) proposes a search direction, e.g. a
+    learned preconditioner's.  It is tried first, scaled to the L-BFGS
+    direction's max |.|, with projected backtracking of at most
+    maxls_learned evaluations; when it is not a descent direction or finds no
+    decrease of f, the plain L-BFGS step is taken instead.  on_step(x_old,
+    g_old, x_new, g_new, learned) sees every accepted step.
+
+    Returns a dict: x, fun, jac, nit, nfev, success, message, n_learned
+    (accepted learned steps), n_fallback (learned proposals rejected)."""
     if x0.dtype != torch.float64 or not x0.is_cuda:
         raise TypeError("x0 must be a float64 device tensor")
     x = x0.clone()
@@ -219,6 +229,7 @@ def minimize(fun: Callable, x0: torch.Tensor, lo: Optional[torch.Tensor] = None,
     d = torch.empty_like(x)
     msg, ok = "maximum number of iterations reached", True
     bounded = lo is not None or hi is not None
+    n_learned = n_fallback = 0
     while nit < maxiter:
         if bounded:
             # quasi-Newton step in the free variables: the binding set is
@@ -243,8 +254,28 @@ def minimize(fun: Callable, x0: torch.Tensor, lo: Optional[torch.Tensor] = None,
         if nit == 0 and len(mem) == 0:
             dn = math.sqrt(dot(d, d))
             t0 = min(1.0, 1.0 / dn) if dn > 0 else 1.0
+        res, learned = None, False
+        if direction_fn is not None:
+            dl = direction_fn(x, g)
+            if dl is not None:
+                dl = dl.to(torch.float64).contiguous()
+                if bounded:
+                    bound_mask(x, g, lo, hi, dl)
+                amax = float(dl.abs().max())
+                if dot(g, dl) < 0 and amax > 0:
+                    ev = _Eval(fun, x, dl, lo, hi)
+                    r = _armijo_projected(ev, f, g, x, t0 * float(d.abs().max()) / amax, c1, maxls_learned)
+                    nfev += ev.nfev
+                    if r is not None and r[1] < f:
+                        res, learned = r, True
+            if learned:
+                n_learned += 1
+            else:
+                n_fallback += 1
         ev = _Eval(fun, x, d, lo, hi)
-        if lo is None and hi is None:
+        if res is not None:
+            pass
+        elif lo is None and hi is None:
             res = _wolfe(ev, f, gd, t0, c1, c2, maxls)
         else:
             res = _armijo_projected(ev, f, g, x, t0, c1, maxls)
@@ -258,8 +289,11 @@ def minimize(fun: Callable, x0: torch.Tensor, lo: Optional[torch.Tensor] = None,
                 break
         _, f_new, g_new, x_new = res
         mem.push(x_new, x, g_new, g)
+        if on_step is not None:
+            on_step(x, g, x_new, g_new, learned)
         x, f, g = x_new, float(f_new), g_new
         nit += 1
         if callback is not None:
             callback(x)
-    return {"x": x, "fun": f, "jac": g, "nit": nit, "nfev": nfev, "success": ok, "message": msg}
+    return {"x": x, "fun": f, "jac": g, "nit": nit, "nfev": nfev, "success": ok, "message": msg,
+            "n_learned": n_learned, "n_fallback": n_fallback}

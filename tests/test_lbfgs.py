@@ -220,3 +220,70 @@ def test_gpu_multiscale_driver_decreases_misfit_and_writes_checkpoints(tmp_path)
         assert {"Vs", "Vp", "rho", "loss_history", "f_c", "cutoff", "invert", "backend"} <= set(z.files)
     with np.load(sd / "epoch_0001.npz") as z:
         assert {"Vs", "Vp", "rho", "iter", "loss"} <= set(z.files)
+
+
+@pytest.mark.gpu
+def test_gpu_failed_learned_proposals_fall_back_to_plain_steps():
+    """A proposed direction that is not a descent direction, or that finds no
+    decrease within its evaluation budget, is rejected: the iterates are then
+    exactly the plain method's."""
+    import torch
+    from stfwi_amd.lbfgs import minimize
+    dev = torch.device("cuda", 0)
+    n = 120
+    A, b, fun = _quadratic(n, 6, dev)
+    x0 = torch.zeros(n, dtype=torch.float64, device=dev)
+    plain = minimize(fun, x0, maxiter=15)
+    ascent = minimize(fun, x0, maxiter=15, direction_fn=lambda x, g: g.clone())
+    assert torch.equal(ascent["x"], plain["x"]) and ascent["n_fallback"] == 15 and ascent["n_learned"] == 0
+    assert ascent["nfev"] == plain["nfev"]
+    # a barely-descent direction (mostly orthogonal to g) with a one-evaluation
+    # budget: at the L-BFGS step length its trial point increases f, so every
+    # proposal falls back
+    v = torch.as_tensor(np.random.default_rng(9).standard_normal(n), device=dev)
+
+    def skew(x, g):
+        orth = v - (v @ g) / (g @ g) * g
+        return -1e-6 * g + orth * (g.norm() / orth.norm())
+    bad = minimize(fun, x0, maxiter=15, maxls_learned=1, direction_fn=skew)
+    assert torch.equal(bad["x"], plain["x"]) and bad["n_fallback"] == 15
+    assert bad["nfev"] == plain["nfev"] + 15
+    # a good proposal (the exact Newton step) is accepted and converges at once
+    Ainv = torch.as_tensor(np.linalg.inv(A), device=dev)
+    newton = minimize(fun, x0, maxiter=3, direction_fn=lambda x, g: -(Ainv @ g))
+    xs = np.linalg.solve(A, b)
+    assert newton["n_learned"] >= 1
+    assert np.linalg.norm(newton["x"].cpu().numpy() - xs) / np.linalg.norm(xs) < 1e-8
+
+
+@pytest.mark.gpu
+def test_gpu_learned_driver_collects_pairs_and_keeps_descending(tmp_path):
+    """Plain stage collecting training pairs, a short training, then a stage
+    with the learned direction: the misfit never increases across accepted
+    steps and every iteration is either a learned step or a fallback."""
+    import torch
+    from stfwi_amd.fwi import forward_jax, ricker_jax
+    from stfwi_amd.fwi.optim_lbfgsb import run_multiscale_fwi
+    from stfwi_amd.learned import LearnedPreconditioner
+    from stfwi_amd.models import anomaly_model
+    nz, nx, pad, nt, dt, dx = 40, 64, 10, 300, 1e-3, 10.0
+    vs_t, vp_t, rho_t = anomaly_model(nz, nx, seed=1)
+    vs0 = np.full_like(vs_t, 1730.0)
+    sx, sz = np.array([10, 32, 54]), np.array([2, 2, 2])
+    rx = np.arange(0, nx, 2)
+    rz = np.full_like(rx, 1)
+    wav = ricker_jax(torch.arange(nt, dtype=torch.float64) * dt, 15.0, 1.5 / 15.0)
+    obs = np.stack([forward_jax(vs_t, vp_t, rho_t, int(a), int(b), rx, rz, nx, nz, dx, dx, dt, nt, 15.0, pad, wav,
+                                fd_order=4, free_surface=True)[2].cpu().numpy() for a, b in zip(sx, sz)])
+    common = dict(Vp_init=vp_t, rho_init=rho_t, src_x=sx, src_z=sz, rec_x=rx, rec_z=rz, nx_dom=nx, nz_dom=nz, dx=dx,
+                  dz=dx, dt=dt, nt=nt, pad=pad, observed=obs, component=2, backend="ad", fd_order=4,
+                  free_surface=True, verbose=False)
+    stage = [dict(f_c=15.0, n_iters=4, invert="vs", illumination_eps=1e-3)]
+    lp = LearnedPreconditioner(nz, nx, seed=0)
+    params, hist = run_multiscale_fwi(Vs_init=vs0, stages=stage, collect=lp, **common)
+    assert len(lp.pairs) == 4
+    lp.train(epochs=60, lr=3e-3, batch=2)
+    params2, hist2 = run_multiscale_fwi(Vs_init=params[0], stages=stage, learned=lp, **common)
+    h = hist2[0]
+    assert h["n_learned"] + h["n_fallback"] == 4
+    assert h["final_loss"] < hist[0]["final_loss"]
