@@ -796,6 +796,10 @@ struct AdjArgs {
     int nrec_max;
     const i64 *uc_cell;                // merged receiver cells
     const int *uc_ptr, *uc_rec;        // CSR of receiver indices per merged cell
+    const int *cellmap;                // [shot][fstride]: merged receiver index of a cell, -1 if none
+    const int *rowcum;                 // [shot][nz + 1]: merged receiver cells in rows < r
+    const T *injm;                     // [shot][nt_max][4][nuc_max]: summed receiver adjoints
+    int nuc_max;
     T *src_adj;                        // [shot][nt_max]
     int imaging, gparams, nacc;
     const T *hist, *hist2;             // [shot][hist_nt][3 | 2][np]
@@ -808,24 +812,65 @@ struct AdjArgs {
     int shot0;                         // first shot of this launch's group
 };
 
-// receiver-adjoint injection at the start of reverse step it
-// (fwi/adjoint_jax_kernel.py:201-206); duplicates accumulate
+// Receiver-adjoint injection (fwi/adjoint_jax_kernel.py:201-206; duplicate
+// receivers accumulate).  The adjoints of each merged receiver cell are
+// summed once per time loop into injm; the time-step kernels then add them:
+// channels vx, vz to U_a, V_a as the strain-adjoint kernel loads its own
+// cells, and channels exx, ezz of step it-1 to the strains the
+// velocity-adjoint kernel of step it stores.  Only a shot's first reverse
+// step (it = nt - 1) needs the separate launch below for exx, ezz.
+template <typename T>
+__global__ void k_merge_inj(AdjArgs<T> a, int nt_run)
+{
+    const int s = blockIdx.z;
+    const ShotDev sh = a.shots[s];
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= sh.nuc) return;
+    const int ui = sh.uc_off + u;
+    const int p0 = a.uc_ptr[ui], p1 = a.uc_ptr[ui + 1];
+    const int tn = sh.nt < nt_run ? sh.nt : nt_run;
+    for (int t = blockIdx.y; t < tn; t += gridDim.y) {
+        const T *ra = a.rec_adj + (i64)s * 4 * a.nt_max * a.nrec_max + (i64)t * a.nrec_max;
+        T *o = (T *)a.injm + (((i64)s * a.nt_max + t) * 4) * a.nuc_max + u;
+        for (int ch = 0; ch < 4; ++ch) {
+            T sum = 0;
+            for (int p = p0; p < p1; ++p) sum += ra[(i64)ch * a.nt_max * a.nrec_max + a.uc_rec[p]];
+            o[(i64)ch * a.nuc_max] = sum;
+        }
+    }
+}
+
+// cellmap[s][origin + cell of merged receiver u] = u (the map is -1 elsewhere)
+__global__ void k_cellmap(const ShotDev *shots, const i64 *uc_cell, i64 fstride, i64 origin, int *cellmap)
+{
+    const int s = blockIdx.y;
+    const ShotDev sh = shots[s];
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= sh.nuc) return;
+    cellmap[s * fstride + origin + uc_cell[sh.uc_off + u]] = u;
+}
+
+// exx, ezz receiver adjoints of a shot's first reverse step
 template <typename T>
 __global__ void k_adj_inject(AdjArgs<T> a, int it)
 {
     const int s = a.shot0 + blockIdx.z;
     const ShotDev sh = a.shots[s];
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (it >= sh.nt || u >= sh.nuc) return;
-    const int ui = sh.uc_off + u;
-    const i64 cell = a.g.origin + a.uc_cell[ui];
+    if (it != sh.nt - 1 || u >= sh.nuc) return;
+    const i64 cell = a.g.origin + a.uc_cell[sh.uc_off + u];
     T *st = a.state + s * a.sstride;
-    const T *ra = a.rec_adj + (i64)s * 4 * a.nt_max * a.nrec_max + (i64)it * a.nrec_max;
-    for (int ch = 0; ch < 4; ++ch) {
-        T sum = 0;
-        for (int p = a.uc_ptr[ui]; p < a.uc_ptr[ui + 1]; ++p) sum += ra[(i64)ch * a.nt_max * a.nrec_max + a.uc_rec[p]];
-        st[ch * a.g.fstride + cell] += sum;
-    }
+    const T *ij = a.injm + (((i64)s * a.nt_max + it) * 4) * a.nuc_max + u;
+    for (int ch = 2; ch < 4; ++ch) st[ch * a.g.fstride + cell] += ij[(i64)ch * a.nuc_max];
+}
+
+// number of merged receiver cells in rows [r0, r0 + tz) of shot s
+template <typename T>
+__device__ __forceinline__ int tile_receivers(const AdjArgs<T> &a, int s, int r0, int tz)
+{
+    const int *rc = a.rowcum + (i64)s * (a.g.nz + 1);
+    const int r1 = r0 + tz < a.g.nz ? r0 + tz : a.g.nz;
+    return rc[r1] - rc[r0];
 }
 
 // adjoints of the four strain-update derivatives at any cell, after the
@@ -1048,6 +1093,23 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
                 }
                 sxV[zi][xi] = xv;
                 sxU[zi][xi] = xu;
+            }
+        }
+    }
+    // receiver adjoints of channels vx, vz of this step into the owned
+    // U_a, V_a (tiles holding receiver cells only)
+    if (tile_receivers(a, s, r0, TZ) > 0) {
+        const int *cm = a.cellmap + (i64)s * g.fstride + g.origin;
+        const T *ij = a.injm + (((i64)s * a.nt_max + it) * 4) * a.nuc_max;
+#pragma unroll
+        for (int rr = 0; rr < RPT; ++rr) {
+            const int r = r0 + ty + BLOCK_Y * rr;
+            if (r < nz && c < nx) {
+                const int u = cm[r * ld + c];
+                if (u >= 0) {
+                    ua0[rr] += ij[u];
+                    va0[rr] += ij[a.nuc_max + u];
+                }
             }
         }
     }
@@ -1274,6 +1336,31 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
             if (a.gparams >= 2) ac1[rr] = ldgg(accb, oa + npi);
         }
     }
+    // receiver adjoints of channels exx, ezz of step it-1, added to the
+    // strains this step stores (receiver tiles only)
+    T inj2[RPT], inj3[RPT];
+    bool hasinj[RPT];
+#pragma unroll
+    for (int rr = 0; rr < RPT; ++rr) {
+        inj2[rr] = inj3[rr] = T(0);
+        hasinj[rr] = false;
+    }
+    if (it >= 1 && tile_receivers(a, s, r0, TZ) > 0) {
+        const int *cm = a.cellmap + (i64)s * g.fstride + g.origin;
+        const T *ij = a.injm + (((i64)s * a.nt_max + it - 1) * 4 + 2) * a.nuc_max;
+#pragma unroll
+        for (int rr = 0; rr < RPT; ++rr) {
+            const int r = r0 + ty + BLOCK_Y * rr;
+            if (r < nz && c < nx) {
+                const int u = cm[r * ld + c];
+                if (u >= 0) {
+                    hasinj[rr] = true;
+                    inj2[rr] = ij[u];
+                    inj3[rr] = ij[a.nuc_max + u];
+                }
+            }
+        }
+    }
     // C-PML inputs of the staged cells and the owned cells' G values, kept
     // for the memory update after the barrier (tiles touching a strip only)
     PmlIn<T> pz[KZ], px[KZ];
@@ -1428,9 +1515,12 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
             const T l2m = l2m0[rr], mu = mu0[rr];
             const T lam = l2m - T(2) * mu;
             const T exa0 = exa0v[rr], eza0 = eza0v[rr];
-            stgg(exa, q, exa0 + l2m * Ax + lam * Az);
-            stgg(eza, q, eza0 + lam * Ax + l2m * Az);
-            stgg(esa, q, esa0v[rr] + T(2) * mu * Mes);
+            const T exan = exa0 + l2m * Ax + lam * Az;
+            const T ezan = eza0 + lam * Ax + l2m * Az;
+            const T esan = esa0v[rr] + T(2) * mu * Mes;
+            stgg(exa, q, hasinj[rr] ? exan + inj2[rr] : exan);
+            stgg(eza, q, hasinj[rr] ? ezan + inj3[rr] : ezan);
+            stgg(esa, q, esan);
 
             // C-PML memories of this cell: psi' = b (G + psi), from the values
             // staged for row rr + 1 of the z-slab
@@ -1454,7 +1544,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
             }
 
             if (a.snap) {
-                const T vals[5] = {Ua[q], Va[q], exa[q], eza[q], esa[q]};
+                const T vals[5] = {Ua[q], Va[q], exan, ezan, esan};
                 int k = 0;
                 for (int f = 0; f < 5; ++f)
                     if (a.snap_mask & (1 << f)) {

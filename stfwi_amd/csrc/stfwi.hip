@@ -86,6 +86,12 @@ struct stfwi_plan {
     ShotDev *shots_d = nullptr;
     i64 *rec_cell_d = nullptr, *uc_cell_d = nullptr;
     int *uc_ptr_d = nullptr, *uc_rec_d = nullptr;
+    // receiver adjoints folded into the adjoint time-step kernels: a map of
+    // every cell to its merged receiver index (-1 if none), per-row prefix
+    // counts of merged receiver cells, and the per-step merged sums
+    int *cellmap_d = nullptr, *rowcum_d = nullptr;
+    void *injm = nullptr;
+    std::vector<int> shot_nt;
     i64 bytes = 0;
     // shots
     int n_shots = 0, nt_run = 0, max_nuc = 0, max_nrec = 0;
@@ -244,6 +250,9 @@ extern "C" int stfwi_plan_create(const stfwi_config *cfg, stfwi_plan **out)
     rc |= dalloc(p, (void **)&p->uc_cell_d, (S * std::max(c.nrec_max, 1)) * sizeof(i64));
     rc |= dalloc(p, (void **)&p->uc_ptr_d, (S * std::max(c.nrec_max, 1) + 1) * sizeof(int));
     rc |= dalloc(p, (void **)&p->uc_rec_d, (S * std::max(c.nrec_max, 1)) * sizeof(int));
+    rc |= dalloc(p, (void **)&p->cellmap_d, S * p->fstride * sizeof(int));
+    rc |= dalloc(p, (void **)&p->rowcum_d, S * (p->nz + 1) * sizeof(int));
+    rc |= dalloc(p, &p->injm, (size_t)S * c.nt_max * 4 * std::max(c.nrec_max, 1) * es);
     if (rc) {
         std::string e = g_err;
         stfwi_plan_destroy(p);
@@ -263,6 +272,8 @@ extern "C" int stfwi_plan_create(const stfwi_config *cfg, stfwi_plan **out)
     hipStream_t st = nullptr;
     if (c.dtype == STFWI_F64) fill<double>(p->mod, 4 * p->fstride + p->nx, st);
     else fill<float>(p->mod, 4 * p->fstride + p->nx, st);
+    // no receivers until stfwi_set_shots
+    fill<int>(p->rowcum_d, S * (p->nz + 1), st);
     HIP_TRY(hipStreamSynchronize(st));
     *out = p;
     return 0;
@@ -310,6 +321,7 @@ extern "C" int stfwi_set_shots(stfwi_plan *p, const stfwi_shot *shots, int n_sho
     std::vector<i64> cells(n_shots * (size_t)std::max(c.nrec_max, 1));
     std::vector<i64> ucell;
     std::vector<int> uptr(1, 0), urec;
+    std::vector<int> rowcum((size_t)n_shots * (p->nz + 1), 0), shot_nt(n_shots);
     int nt_run = 0, max_nuc = 0, max_nrec = 0;
     for (int s = 0; s < n_shots; ++s) {
         const stfwi_shot &h = shots[s];
@@ -337,12 +349,16 @@ extern "C" int stfwi_set_shots(stfwi_plan *p, const stfwi_shot *shots, int n_sho
             cells[(size_t)s * c.nrec_max + j] = cell;
             byc[cell].push_back(j);
         }
+        int *rcum = rowcum.data() + (size_t)s * (p->nz + 1);
         for (auto &kv : byc) {
             ucell.push_back(kv.first);
             for (int j : kv.second) urec.push_back(j);
             uptr.push_back((int)urec.size());
+            ++rcum[kv.first / p->ld + 1];
         }
+        for (int r = 0; r < p->nz; ++r) rcum[r + 1] += rcum[r];
         d.nuc = (int)ucell.size() - d.uc_off;
+        shot_nt[s] = h.nt;
         nt_run = std::max(nt_run, h.nt);
         max_nuc = std::max(max_nuc, d.nuc);
         max_nrec = std::max(max_nrec, h.nrec);
@@ -353,6 +369,15 @@ extern "C" int stfwi_set_shots(stfwi_plan *p, const stfwi_shot *shots, int n_sho
     HIP_TRY(hipMemcpyAsync(p->uc_cell_d, ucell.data(), ucell.size() * sizeof(i64), hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(p->uc_ptr_d, uptr.data(), uptr.size() * sizeof(int), hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(p->uc_rec_d, urec.data(), urec.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(p->rowcum_d, rowcum.data(), rowcum.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    {
+        const i64 n = (i64)n_shots * p->fstride;
+        hipLaunchKernelGGL(k_fill<int>, dim3((int)std::min<i64>((n + 255) / 256, 65536)), dim3(256), 0, st,
+                           p->cellmap_d, n, -1);
+        hipLaunchKernelGGL(k_cellmap, dim3((max_nuc + 127) / 128, n_shots), dim3(128), 0, st,
+                           (const ShotDev *)p->shots_d, p->uc_cell_d, p->fstride, p->origin, p->cellmap_d);
+        HIP_TRY(hipGetLastError());
+    }
     // the wavelets are read by the kernels of every later time loop: keep a copy
     HIP_TRY(hipMemcpyAsync(p->wav_int, wavelets, (size_t)n_shots * c.nt_max * p->esz, hipMemcpyDeviceToDevice, st));
     // the host vectors die at return: wait for the copies
@@ -361,6 +386,7 @@ extern "C" int stfwi_set_shots(stfwi_plan *p, const stfwi_shot *shots, int n_sho
     p->nt_run = nt_run;
     p->max_nuc = max_nuc;
     p->max_nrec = max_nrec;
+    p->shot_nt = shot_nt;
     p->wav = p->wav_int;
     p->fwd_done = false;
     return 0;
@@ -503,6 +529,8 @@ static AdjArgs<T> adj_args(stfwi_plan *p, const void *rec_adj, void *src_adj, vo
     a.wav = (const T *)p->wav; a.nt_max = c.nt_max;
     a.rec_adj = (const T *)rec_adj; a.nrec_max = c.nrec_max;
     a.uc_cell = p->uc_cell_d; a.uc_ptr = p->uc_ptr_d; a.uc_rec = p->uc_rec_d;
+    a.cellmap = p->cellmap_d; a.rowcum = p->rowcum_d; a.injm = (const T *)p->injm;
+    a.nuc_max = std::max(c.nrec_max, 1);
     a.src_adj = (T *)src_adj;
     a.imaging = p->hist ? c.imaging : 0;
     a.gparams = c.grad_params; a.nacc = p->nacc;
@@ -690,6 +718,9 @@ static int adjoint_t(stfwi_plan *p, const void *rec_adj, void *src_adj, void *sn
         fill<T>(p->psiz_a[k], (i64)S * 4 * p->psiz_len, st);
     }
     if (a.imaging) fill<T>(p->gacc, (i64)S * p->nacc * p->np, st);
+    // merged receiver adjoints of every step, read by the time-step kernels
+    hipLaunchKernelGGL(k_merge_inj<T>, dim3((p->max_nuc + 127) / 128, std::min(p->nt_run, 65535), S), dim3(128), 0,
+                       st, a, p->nt_run);
     int g0[STFWI_MAX_GROUPS + 1];
     const int G = group_bounds(p, g0);
     hipStream_t gs[STFWI_MAX_GROUPS];
@@ -734,7 +765,11 @@ static int adjoint_t(stfwi_plan *p, const void *rec_adj, void *src_adj, void *sn
             for (int k = 0; k < G; ++k) {
                 const int ns = g0[k + 1] - g0[k];
                 a.shot0 = g0[k];
-                hipLaunchKernelGGL(k_adj_inject<T>, dim3((p->max_nuc + 63) / 64, 1, ns), dim3(64), 0, gs[k], a, it);
+                bool first = false;
+                for (int s = g0[k]; s < g0[k + 1]; ++s) first |= p->shot_nt[s] - 1 == it;
+                if (first)
+                    hipLaunchKernelGGL(k_adj_inject<T>, dim3((p->max_nuc + 63) / 64, 1, ns), dim3(64), 0, gs[k], a,
+                                       it);
                 launch_adj<T, FD>(2, p->tz[2], a, ns, gs[k], it);
                 launch_adj<T, FD>(3, p->tz[3], a, ns, gs[k], it);
             }
