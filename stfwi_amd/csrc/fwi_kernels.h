@@ -1719,80 +1719,98 @@ __device__ __forceinline__ T taper_w(int t, int nt, int n_taper)
     return (T)((cos((double)k / n * M_PI) + 1.0) / 2.0);
 }
 
+// one thread per (shot, receiver): the component channel's residual, its
+// transpose in place, and the receiver's sum of squares in lpart
+// [shot][nrec_max] (0 for absent receivers)
 template <typename T>
-__global__ __launch_bounds__(256) void k_residual(ResArgs<T> a)
+__global__ __launch_bounds__(64) void k_residual(ResArgs<T> a, double *lpart)
 {
-    const int s = blockIdx.x;
+    const int s = blockIdx.y;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= a.nrec_max) return;
     const ShotDev sh = a.shots[s];
     const int nt = sh.nt;
     const i64 chs = (i64)a.nt_max * a.nrec_max;
     const T *pred = a.pred + ((i64)s * 4 + a.comp) * chs;
     const T *obs = a.obs + (i64)s * chs;
-    T *adj = a.rec_adj + (i64)s * 4 * chs;
+    T *rr = a.rec_adj + ((i64)s * 4 + a.comp) * chs + j;
+    if (j >= sh.nrec) {
+        for (int t = 0; t < a.nt_max; ++t) rr[(i64)t * a.nrec_max] = 0;
+        lpart[(i64)s * a.nrec_max + j] = 0;
+        return;
+    }
+    // forward: r = F T pred - F T obs, stored in place
+    T xs[MAX_SOS][2][2], ys[MAX_SOS][2][2];   // [section][pred/obs][lag]
+    for (int k = 0; k < a.n_sos; ++k)
+        for (int u = 0; u < 2; ++u) xs[k][u][0] = xs[k][u][1] = ys[k][u][0] = ys[k][u][1] = 0;
+    T ssq = 0;
+    for (int t = 0; t < nt; ++t) {
+        const T tw = taper_w<T>(t, nt, a.n_taper);
+        T v[2] = {pred[(i64)t * a.nrec_max + j], obs[(i64)t * a.nrec_max + j]};
+        for (int u = 0; u < 2; ++u) {
+            if (a.n_taper > 0) v[u] = v[u] * tw;
+            for (int k = 0; k < a.n_sos; ++k) {
+                const T *cf = a.coef[k];
+                const T y = cf[0] * v[u] + cf[1] * xs[k][u][0] + cf[2] * xs[k][u][1] - cf[3] * ys[k][u][0] - cf[4] * ys[k][u][1];
+                xs[k][u][1] = xs[k][u][0];
+                xs[k][u][0] = v[u];
+                ys[k][u][1] = ys[k][u][0];
+                ys[k][u][0] = y;
+                v[u] = y;
+            }
+        }
+        const T res = v[0] - v[1];
+        rr[(i64)t * a.nrec_max] = res;
+        ssq += res * res;
+    }
+    for (int t = nt; t < a.nt_max; ++t) rr[(i64)t * a.nrec_max] = 0;
+    lpart[(i64)s * a.nrec_max + j] = (double)ssq;
+    // transpose: cot = 2 r w / N, then F^T (reverse-time recursion), T^T
+    const T g0 = T(2) * (a.w / (T)((double)nt * sh.nrec));
+    T cx1[MAX_SOS], cx2[MAX_SOS], cy1[MAX_SOS], cy2[MAX_SOS];
+    for (int k = 0; k < a.n_sos; ++k) cx1[k] = cx2[k] = cy1[k] = cy2[k] = 0;
+    for (int t = nt - 1; t >= 0; --t) {
+        T gcur = rr[(i64)t * a.nrec_max] * g0;
+        for (int k = a.n_sos - 1; k >= 0; --k) {
+            const T *cf = a.coef[k];
+            const T gy = gcur + cy1[k];
+            const T gx = cx1[k] + cf[0] * gy;
+            cx1[k] = cx2[k] + cf[1] * gy;
+            cx2[k] = cf[2] * gy;
+            cy1[k] = cy2[k] - cf[3] * gy;
+            cy2[k] = -cf[4] * gy;
+            gcur = gx;
+        }
+        if (a.n_taper > 0) gcur = gcur * taper_w<T>(t, nt, a.n_taper);
+        rr[(i64)t * a.nrec_max] = gcur;
+    }
+}
+
+// the other three channels of the receiver adjoints are zero
+template <typename T>
+__global__ void k_residual_zero(T *rec_adj, i64 n, i64 chs, int comp)
+{
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
+        if ((i / chs) % 4 != comp) rec_adj[i] = 0;
+}
+
+// per-shot misfit: thread t sums receivers t, t + 256, ... in order, then a
+// fixed tree (one block per shot)
+template <typename T>
+__global__ __launch_bounds__(256) void k_residual_loss(ResArgs<T> a, const double *lpart)
+{
+    const int s = blockIdx.x;
+    const ShotDev sh = a.shots[s];
     __shared__ double red[256];
     double part = 0;
-    for (int j = threadIdx.x; j < a.nrec_max; j += blockDim.x) {
-        for (int ch = 0; ch < 4; ++ch)
-            if (ch != a.comp)
-                for (int t = 0; t < a.nt_max; ++t) adj[ch * chs + (i64)t * a.nrec_max + j] = 0;
-        T *rr = adj + a.comp * chs + j;
-        if (j >= sh.nrec) {
-            for (int t = 0; t < a.nt_max; ++t) rr[(i64)t * a.nrec_max] = 0;
-            continue;
-        }
-        // forward: r = F T pred - F T obs, stored in place
-        T xs[MAX_SOS][2][2], ys[MAX_SOS][2][2];   // [section][pred/obs][lag]
-        for (int k = 0; k < a.n_sos; ++k)
-            for (int u = 0; u < 2; ++u) xs[k][u][0] = xs[k][u][1] = ys[k][u][0] = ys[k][u][1] = 0;
-        T ssq = 0;
-        for (int t = 0; t < nt; ++t) {
-            const T tw = taper_w<T>(t, nt, a.n_taper);
-            T v[2] = {pred[(i64)t * a.nrec_max + j], obs[(i64)t * a.nrec_max + j]};
-            for (int u = 0; u < 2; ++u) {
-                if (a.n_taper > 0) v[u] = v[u] * tw;
-                for (int k = 0; k < a.n_sos; ++k) {
-                    const T *cf = a.coef[k];
-                    const T y = cf[0] * v[u] + cf[1] * xs[k][u][0] + cf[2] * xs[k][u][1] - cf[3] * ys[k][u][0] - cf[4] * ys[k][u][1];
-                    xs[k][u][1] = xs[k][u][0];
-                    xs[k][u][0] = v[u];
-                    ys[k][u][1] = ys[k][u][0];
-                    ys[k][u][0] = y;
-                    v[u] = y;
-                }
-            }
-            const T res = v[0] - v[1];
-            rr[(i64)t * a.nrec_max] = res;
-            ssq += res * res;
-        }
-        for (int t = nt; t < a.nt_max; ++t) rr[(i64)t * a.nrec_max] = 0;
-        part += (double)ssq;
-        // transpose: cot = 2 r w / N, then F^T (reverse-time recursion), T^T
-        const T g0 = T(2) * (a.w / (T)((double)nt * sh.nrec));
-        T cx1[MAX_SOS], cx2[MAX_SOS], cy1[MAX_SOS], cy2[MAX_SOS];
-        for (int k = 0; k < a.n_sos; ++k) cx1[k] = cx2[k] = cy1[k] = cy2[k] = 0;
-        for (int t = nt - 1; t >= 0; --t) {
-            T gcur = rr[(i64)t * a.nrec_max] * g0;
-            for (int k = a.n_sos - 1; k >= 0; --k) {
-                const T *cf = a.coef[k];
-                const T gy = gcur + cy1[k];
-                const T gx = cx1[k] + cf[0] * gy;
-                cx1[k] = cx2[k] + cf[1] * gy;
-                cx2[k] = cf[2] * gy;
-                cy1[k] = cy2[k] - cf[3] * gy;
-                cy2[k] = -cf[4] * gy;
-                gcur = gx;
-            }
-            if (a.n_taper > 0) gcur = gcur * taper_w<T>(t, nt, a.n_taper);
-            rr[(i64)t * a.nrec_max] = gcur;
-        }
-    }
+    for (int j = threadIdx.x; j < a.nrec_max; j += blockDim.x) part += lpart[(i64)s * a.nrec_max + j];
     red[threadIdx.x] = part;
     __syncthreads();
     for (int k = 128; k > 0; k >>= 1) {
         if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
         __syncthreads();
     }
-    if (threadIdx.x == 0) a.loss[s] = (T)((double)a.w * (red[0] / ((double)nt * sh.nrec)));
+    if (threadIdx.x == 0) a.loss[s] = (T)((double)a.w * (red[0] / ((double)sh.nt * sh.nrec)));
 }
 
 }  // namespace stfwi

@@ -90,6 +90,7 @@ struct stfwi_plan {
     // every cell to its merged receiver index (-1 if none), per-row prefix
     // counts of merged receiver cells, and the per-step merged sums
     int *cellmap_d = nullptr, *rowcum_d = nullptr;
+    double *lpart = nullptr;   // [shot][nrec_max] receiver sums of squares of the residual
     void *injm = nullptr;
     std::vector<int> shot_nt;
     i64 bytes = 0;
@@ -252,6 +253,7 @@ extern "C" int stfwi_plan_create(const stfwi_config *cfg, stfwi_plan **out)
     rc |= dalloc(p, (void **)&p->uc_rec_d, (S * std::max(c.nrec_max, 1)) * sizeof(int));
     rc |= dalloc(p, (void **)&p->cellmap_d, S * p->fstride * sizeof(int));
     rc |= dalloc(p, (void **)&p->rowcum_d, S * (p->nz + 1) * sizeof(int));
+    rc |= dalloc(p, (void **)&p->lpart, S * std::max(c.nrec_max, 1) * sizeof(double));
     rc |= dalloc(p, &p->injm, (size_t)S * c.nt_max * 4 * std::max(c.nrec_max, 1) * es);
     if (rc) {
         std::string e = g_err;
@@ -858,7 +860,11 @@ static int residual_t(stfwi_plan *p, const void *pred, const void *obs, int comp
         a.coef[k][3] = (T)(s[4] / s[3]); a.coef[k][4] = (T)(s[5] / s[3]);
     }
     a.w = (T)(scale / n_norm);
-    hipLaunchKernelGGL(k_residual<T>, dim3(p->n_shots), dim3(256), 0, st, a);
+    const i64 chs = (i64)a.nt_max * a.nrec_max, n = (i64)p->n_shots * 4 * chs;
+    hipLaunchKernelGGL(k_residual_zero<T>, dim3((int)std::min<i64>((n + 255) / 256, 16384)), dim3(256), 0, st,
+                       (T *)rec_adj, n, chs, comp);
+    hipLaunchKernelGGL(k_residual<T>, dim3((a.nrec_max + 63) / 64, p->n_shots), dim3(64), 0, st, a, p->lpart);
+    hipLaunchKernelGGL(k_residual_loss<T>, dim3(p->n_shots), dim3(256), 0, st, a, (const double *)p->lpart);
     HIP_TRY(hipGetLastError());
     return 0;
 }
