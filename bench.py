@@ -30,7 +30,7 @@ HBM_PEAK = 8.0e12
 # from rocprofv3 FETCH_SIZE and WRITE_SIZE passes over
 # `bench.py --kernels-only --tiles 8,8,8,4`, scaled by the counters' readings
 # for a 256 MiB device copy (reads x 2.00, writes x 1.00).
-ADJ_VEL_TRAFFIC = {4: (51309.6 * 2.0 + 38341.2) * 1024}
+ADJ_VEL_TRAFFIC = {4: (51629.7 * 2.0 + 38313.0) * 1024}
 
 
 def algorithmic_bytes_per_shot_gradient(s=4, k=1, nt=NT):
@@ -107,6 +107,54 @@ def cpu_baseline(sv, budget_s=15.0):
                       f"152x500 fd8 workload, scaled to nt={NT}; {el:.1f} s"}
 
 
+def learned_bench(reps=50):
+    """The learned preconditioner's matrix-core kernels at the workload's
+    grid: one 3x3 convolution layer (32 -> 32 channels, batch 4, 152x500) in
+    its forward, data-gradient and weight-gradient forms, timed with HIP
+    events on the launch stream; flops = 2 B H W Cin Cout 9 per form, against
+    the 2.5 PFLOP/s dense BF16 peak.  Then one training epoch of the
+    preconditioner on 8 pairs."""
+    import torch
+    from stfwi_amd import learned as L
+    dev = torch.device("cuda", 0)
+    g = torch.Generator().manual_seed(0)
+    B, ci, co, h, w = 4, 32, 32, NZ, NX
+    x = torch.randn((B, ci, h, w), generator=g).to(dev)
+    wt = (torch.randn((co, ci, 3, 3), generator=g) * 0.1).to(dev)
+    bias = torch.zeros(co, device=dev)
+    gy = torch.randn((B, co, h, w), generator=g).to(dev)
+    flops = 2.0 * B * h * w * ci * co * 9
+    forms = {"conv_fwd": lambda: L.conv(x, wt, bias, relu=True),
+             "conv_dgrad": lambda: L.conv_dgrad(gy, wt),
+             "conv_wgrad": lambda: L.conv_wgrad(x, gy, wt.shape)}
+    out = {}
+    for name, f in forms.items():
+        f()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            f()
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / reps
+        out[name] = {"us": round(us, 2), "TFLOP/s": round(flops / us / 1e6, 1),
+                     "frac_of_bf16_peak": round(flops / us / 1e6 / 2500.0, 4)}
+    lp = L.LearnedPreconditioner(NZ, NX, seed=1)
+    rng = np.random.default_rng(0)
+    for _ in range(8):
+        gr = rng.standard_normal((NZ, NX))
+        lp.add_pair(gr, np.ones((NZ, NX)), -gr)
+    lp.train(epochs=1, lr=1e-3, batch=4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lp.train(epochs=5, lr=1e-3, batch=4)
+    torch.cuda.synchronize()
+    out["train_epoch_ms"] = round((time.perf_counter() - t0) / 5 * 1e3, 2)
+    out["layer"] = f"batch {B}, {ci}->{co} channels, 3x3, {h}x{w}, BF16 operands, FP32 accumulation"
+    return out
+
+
 def main():
     global NSHOT
     ap = argparse.ArgumentParser()
@@ -120,6 +168,9 @@ def main():
     ap.add_argument("--groups", type=int, default=3, help="concurrent stream groups of shots (1-4)")
     ap.add_argument("--tiles", default=None,
                     help="fixed tile heights of the four time-step kernels, e.g. 8,8,8,4 (default: autotune)")
+    ap.add_argument("--learned", action="store_true",
+                    help="time the learned preconditioner's matrix-core convolution kernels and one "
+                         "training epoch instead of the gradient path")
     ap.add_argument("--kernels-only", action="store_true",
                     help="after one forward, time each time-step kernel (20 launches) and a "
                          "256 MiB device copy; for hardware-counter runs")
@@ -133,6 +184,9 @@ def main():
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.learned:
+        print(json.dumps({"learned": learned_bench()}))
+        return
     NSHOT = args.shots
     sv = survey(0)
     f32 = torch.float32

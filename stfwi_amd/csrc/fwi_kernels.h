@@ -1731,9 +1731,9 @@ __global__ __launch_bounds__(64) void k_residual(ResArgs<T> a, double *lpart)
     const ShotDev sh = a.shots[s];
     const int nt = sh.nt;
     const i64 chs = (i64)a.nt_max * a.nrec_max;
-    const T *pred = a.pred + ((i64)s * 4 + a.comp) * chs;
-    const T *obs = a.obs + (i64)s * chs;
-    T *rr = a.rec_adj + ((i64)s * 4 + a.comp) * chs + j;
+    const T *__restrict__ pred = a.pred + ((i64)s * 4 + a.comp) * chs;
+    const T *__restrict__ obs = a.obs + (i64)s * chs;
+    T *__restrict__ rr = a.rec_adj + ((i64)s * 4 + a.comp) * chs + j;
     if (j >= sh.nrec) {
         for (int t = 0; t < a.nt_max; ++t) rr[(i64)t * a.nrec_max] = 0;
         lpart[(i64)s * a.nrec_max + j] = 0;
@@ -1744,6 +1744,7 @@ __global__ __launch_bounds__(64) void k_residual(ResArgs<T> a, double *lpart)
     for (int k = 0; k < a.n_sos; ++k)
         for (int u = 0; u < 2; ++u) xs[k][u][0] = xs[k][u][1] = ys[k][u][0] = ys[k][u][1] = 0;
     T ssq = 0;
+#pragma unroll 4
     for (int t = 0; t < nt; ++t) {
         const T tw = taper_w<T>(t, nt, a.n_taper);
         T v[2] = {pred[(i64)t * a.nrec_max + j], obs[(i64)t * a.nrec_max + j]};

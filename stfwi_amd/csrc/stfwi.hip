@@ -948,19 +948,27 @@ extern "C" int stfwi_autotune(stfwi_plan *p, int reps, void *stream, int *tz_out
     const int cand[4] = {4, 8, 16, 32};
     const bool f64 = p->cfg.dtype == STFWI_F64;
     for (int which = 0; which < 4; ++which) {
+        // three interleaved rounds, the fastest round of each candidate: a
+        // transient slowdown during one round does not decide the choice
+        double t[4] = {1e300, 1e300, 1e300, 1e300};
+        for (int round = 0; round < 3; ++round)
+            for (int ci = 0; ci < 4; ++ci) {
+                const int tz = cand[ci];
+                if (f64 ? !tz_supported<double>(tz) : !tz_supported<float>(tz)) continue;
+                double ms = 0;
+                int rc = time_kernel(p, which, tz, 2, (hipStream_t)stream, &ms);   // warm-up
+                if (!rc) rc = time_kernel(p, which, tz, reps, (hipStream_t)stream, &ms);
+                if (rc) return rc;
+                t[ci] = std::min(t[ci], ms);
+            }
         double best = 1e300;
         int best_tz = p->tz[which];
         for (int ci = 0; ci < 4; ++ci) {
-            const int tz = cand[ci];
-            if (f64 ? !tz_supported<double>(tz) : !tz_supported<float>(tz)) continue;
-            double ms = 0;
-            int rc = time_kernel(p, which, tz, 2, (hipStream_t)stream, &ms);   // warm-up
-            if (!rc) rc = time_kernel(p, which, tz, reps, (hipStream_t)stream, &ms);
-            if (rc) return rc;
-            if (us_out) us_out[which * 4 + ci] = ms * 1e3 / reps;
-            if (ms < best) {
-                best = ms;
-                best_tz = tz;
+            if (t[ci] >= 1e300) continue;
+            if (us_out) us_out[which * 4 + ci] = t[ci] * 1e3 / reps;
+            if (t[ci] < best) {
+                best = t[ci];
+                best_tz = cand[ci];
             }
         }
         p->tz[which] = best_tz;
