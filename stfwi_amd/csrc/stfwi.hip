@@ -1201,6 +1201,40 @@ extern "C" int stfwi_set_graphs(stfwi_plan *p, int enable)
 // ---------------------------------------------------------------------------
 static int grid1(long long n) { return (int)std::min<long long>((n + 255) / 256, 8192); }
 
+// the LDS-tiled convolution (input channels of the call <= 64): the weights
+// are packed once per call into a stream-ordered pool allocation, then every
+// block copies its slice.  Returns false when the shape needs the untiled
+// kernel.
+template <int KS, bool TR>
+static bool conv_tile(const float *in, int batch, int cin, int h, int w, const float *wt, const float *bias, int cout,
+                      float *out, int relu, hipStream_t st)
+{
+    if (cin > 64) return false;
+    const int cinp = (cin + 7) / 8 * 8, Kp = (KS * KS * cinp + 31) / 32 * 32;
+    int ps, wps;
+    conv_strides(cinp, Kp, ps, wps);
+    const int PR = CONV_TROWS + KS - 1, PC = 64 + KS - 1;
+    const size_t lds = (size_t)PR * PC * ps * 2 + (size_t)32 * wps * 2 + (size_t)(Kp / 8) * 4;
+    static bool raised = false;   // the 64 KiB default limit raised once per kernel
+    if (lds > 65536 && !raised) {
+        if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_conv_tile<KS>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess)
+            return false;
+        raised = true;
+    }
+    const int nct = (cout + 31) / 32;
+    const long long n = (long long)nct * 32 * wps;
+    void *wpk = nullptr;
+    if (hipMallocAsync(&wpk, (size_t)n * 2, st) != hipSuccess) return false;
+    hipLaunchKernelGGL((k_conv_pack<KS, TR>), dim3(grid1(n)), dim3(256), 0, st, wt, cin, cout, cinp, wps,
+                       (__bf16 *)wpk, n);
+    dim3 grid((unsigned)((w + 63) / 64), (unsigned)((h + CONV_TROWS - 1) / CONV_TROWS), (unsigned)(batch * nct));
+    hipLaunchKernelGGL((k_conv_tile<KS>), grid, dim3(256), lds, st, in, cin, h, w, (const __bf16 *)wpk, bias, cout,
+                       out, relu, cinp, Kp);
+    (void)hipFreeAsync(wpk, st);
+    return true;
+}
+
 extern "C" int stfwi_conv2d_bf16(const float *in, int batch, int cin, int h, int w, const float *weight,
                                  const float *bias, int cout, int ksize, int relu, float *out, void *stream)
 {
@@ -1209,13 +1243,18 @@ extern "C" int stfwi_conv2d_bf16(const float *in, int batch, int cin, int h, int
     if (ksize != 1 && ksize != 3) return fail(STFWI_ERR_UNSUPPORTED, "ksize=%d (use 1 or 3)", ksize);
     const long long P = (long long)h * w;
     if (P > (1LL << 30)) return fail(STFWI_ERR_INVALID, "image too large");
+    hipStream_t st = (hipStream_t)stream;
+    const bool tiled = ksize == 3 ? conv_tile<3, false>(in, batch, cin, h, w, weight, bias, cout, out, relu, st)
+                                  : conv_tile<1, false>(in, batch, cin, h, w, weight, bias, cout, out, relu, st);
     dim3 grid((unsigned)((P + 255) / 256), (unsigned)((cout + 15) / 16), (unsigned)batch);
-    if (ksize == 3)
-        hipLaunchKernelGGL((k_conv_mfma<3, false>), grid, dim3(256), 0, (hipStream_t)stream, in, cin, h, w, weight,
-                           bias, cout, out, relu);
-    else
-        hipLaunchKernelGGL((k_conv_mfma<1, false>), grid, dim3(256), 0, (hipStream_t)stream, in, cin, h, w, weight,
-                           bias, cout, out, relu);
+    if (tiled) {
+    } else if (ksize == 3) {
+        hipLaunchKernelGGL((k_conv_mfma<3, false>), grid, dim3(256), 0, st, in, cin, h, w, weight, bias, cout, out,
+                           relu);
+    } else {
+        hipLaunchKernelGGL((k_conv_mfma<1, false>), grid, dim3(256), 0, st, in, cin, h, w, weight, bias, cout, out,
+                           relu);
+    }
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -1227,15 +1266,20 @@ extern "C" int stfwi_conv2d_bf16_dgrad(const float *dout, int batch, int cout, i
         return fail(STFWI_ERR_INVALID, "bad argument");
     if (ksize != 1 && ksize != 3) return fail(STFWI_ERR_UNSUPPORTED, "ksize=%d (use 1 or 3)", ksize);
     const long long P = (long long)h * w;
-    dim3 grid((unsigned)((P + 255) / 256), (unsigned)((cin + 15) / 16), (unsigned)batch);
     // a convolution of dout (cout channels) into cin channels with the
     // transposed, flipped weights
-    if (ksize == 3)
-        hipLaunchKernelGGL((k_conv_mfma<3, true>), grid, dim3(256), 0, (hipStream_t)stream, dout, cout, h, w, weight,
-                           nullptr, cin, din, 0);
-    else
-        hipLaunchKernelGGL((k_conv_mfma<1, true>), grid, dim3(256), 0, (hipStream_t)stream, dout, cout, h, w, weight,
-                           nullptr, cin, din, 0);
+    hipStream_t st = (hipStream_t)stream;
+    const bool tiled = ksize == 3 ? conv_tile<3, true>(dout, batch, cout, h, w, weight, nullptr, cin, din, 0, st)
+                                  : conv_tile<1, true>(dout, batch, cout, h, w, weight, nullptr, cin, din, 0, st);
+    dim3 grid((unsigned)((P + 255) / 256), (unsigned)((cin + 15) / 16), (unsigned)batch);
+    if (tiled) {
+    } else if (ksize == 3) {
+        hipLaunchKernelGGL((k_conv_mfma<3, true>), grid, dim3(256), 0, st, dout, cout, h, w, weight, nullptr, cin,
+                           din, 0);
+    } else {
+        hipLaunchKernelGGL((k_conv_mfma<1, true>), grid, dim3(256), 0, st, dout, cout, h, w, weight, nullptr, cin,
+                           din, 0);
+    }
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -1256,23 +1300,25 @@ extern "C" int stfwi_conv2d_bf16_wgrad(const float *in, const float *dout, int b
     if (work_elems < stfwi_conv2d_bf16_wgrad_work(cin, cout, ksize))
         return fail(STFWI_ERR_INVALID, "work buffer holds %lld elements, needs %lld", work_elems,
                     stfwi_conv2d_bf16_wgrad_work(cin, cout, ksize));
-    // up to 256 pixel chunks of a multiple of 32 pixels
-    const long long pch = 32 * ((PT + 32 * 256 - 1) / (32 * 256));
-    const int nz = (int)((PT + pch - 1) / pch);
+    // up to 256 chunks of whole image rows
     hipStream_t st = (hipStream_t)stream;
+    const int rows = batch * h, rg = (rows + 255) / 256;
+    const int nz = (rows + rg - 1) / rg;
     dim3 grid((unsigned)((K + 15) / 16), (unsigned)((cout + 15) / 16), (unsigned)nz);
     if (ksize == 3)
-        hipLaunchKernelGGL(k_conv_wgrad_mfma<3>, grid, dim3(64), 0, st, in, cin, h, w, dout, cout, batch, (int)pch,
-                           work);
+        hipLaunchKernelGGL(k_conv_wgrad_rows<3>, grid, dim3(64), 0, st, in, cin, h, w, dout, cout, batch, rg, work);
     else
-        hipLaunchKernelGGL(k_conv_wgrad_mfma<1>, grid, dim3(64), 0, st, in, cin, h, w, dout, cout, batch, (int)pch,
-                           work);
+        hipLaunchKernelGGL(k_conv_wgrad_rows<1>, grid, dim3(64), 0, st, in, cin, h, w, dout, cout, batch, rg, work);
     const long long n = (long long)cout * K;
     hipLaunchKernelGGL(k_sum_chunks, dim3(grid1(n)), dim3(256), 0, st, work, nz, n, dw, accumulate);
     if (db) {
-        // bias gradient: per-channel sums into work, then into db
-        hipLaunchKernelGGL(k_bias_grad, dim3(cout), dim3(256), 0, st, dout, cout, batch, h * w, work);
-        hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(256), 0, st, work, 1, (long long)cout, db, accumulate);
+        // bias gradient: per-chunk channel sums into work (after the
+        // weight partials are summed), then summed in chunk order into db
+        const long long bch = 4096;
+        const int nb = (int)std::min<long long>((PT + bch - 1) / bch, 256);
+        const long long pchb = (PT + nb - 1) / nb;
+        hipLaunchKernelGGL(k_bias_grad, dim3(cout, nb), dim3(256), 0, st, dout, cout, batch, h * w, (int)pchb, work);
+        hipLaunchKernelGGL(k_sum_chunks, dim3(grid1(cout)), dim3(256), 0, st, work, nb, (long long)cout, db, accumulate);
     }
     HIP_TRY(hipGetLastError());
     return 0;
