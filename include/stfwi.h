@@ -171,6 +171,24 @@ int stfwi_eprecond3(const void *ws, int n_shots, int nz_t, int nx_t, int free_su
                     int nx_dom, const int *rec_x, const int *rec_z, int nrec, double dx, double dz, double eps,
                     int dtype, void *we_pad, void *out, void *stream);
 
+/* Shin energy weight We = sqrt(Ws Wr) + eps max(sqrt(Ws Wr)) over n elements
+ * (eprecond1, fwi/preconditioning.py:37-62); dtype of ws, wr and we. */
+int stfwi_eprecond1(const void *ws, const void *wr, long long n, double eps, int dtype, void *we, void *stream);
+
+/* Depth taper of the gradient, FP64 (nz x nx), rows as taper_grad
+ * (fwi/preconditioning.py:144-183): 0 above gradt1, exp(-(gradt2-j)^2) up to
+ * gradt2, (j dz)^exp_taper below; with bottom != 0 the roll-off between gradb1
+ * and gradb2 and 0 below gradb2. */
+int stfwi_taper_grad(int nz, int nx, double dz, int gradt1, int gradt2, double exp_taper, int bottom, int gradb1,
+                     int gradb2, double *out, void *stream);
+
+/* Source taper, FP64 (nz x nx), as build_source_taper (fwi/taper.py:16-84):
+ * the erf patch of radius srt_radius stamped (minimum) around every source,
+ * normalised to [0, 1], then a (2 filt_size + 1)^2 zero box at each source.
+ * src_xz is a host array of nsrc (x, z) grid-index pairs.  Host-synchronous. */
+int stfwi_source_taper(int nz, int nx, double dh, const int *src_xz, int nsrc, double srt_radius, int filt_size,
+                       double *out, void *stream);
+
 /* In-place trace filtering x[t][trace] (nt x ntr): taper the last n_taper
  * samples then the DF-I biquad cascade (jax_sosfilt / cosine_taper_end,
  * fwi/filters.py:14-42), or with transpose=1 the exact transpose of that

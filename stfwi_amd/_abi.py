@@ -41,6 +41,10 @@ SIGNATURES = {
                                      C.c_double, C.c_int, _VP, _VP]),
     "stfwi_eprecond3": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _VP, _VP,
                                   C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _VP, _VP, _VP]),
+    "stfwi_eprecond1": (C.c_int, [_VP, _VP, C.c_longlong, C.c_double, C.c_int, _VP, _VP]),
+    "stfwi_taper_grad": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
+                                   C.c_int, _VP, _VP]),
+    "stfwi_source_taper": (C.c_int, [C.c_int, C.c_int, C.c_double, _VP, C.c_int, C.c_double, C.c_int, _VP, _VP]),
     "stfwi_sosfilt": (C.c_int, [_VP, C.c_int, C.c_int, _VP, C.c_int, C.c_int, C.c_int, C.c_int, _VP]),
     "stfwi_zhou_imaging": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, C.c_int, C.c_int, C.c_int,
                                      C.c_double, C.c_double, C.c_double, C.c_int, _VP, _VP]),

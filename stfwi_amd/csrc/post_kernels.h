@@ -237,4 +237,138 @@ __global__ void k_zhou_cubes(const T *__restrict__ ex, const T *__restrict__ ez,
     out[2 * N + q] = -dt * gr;
 }
 
+// ---------------------------------------------------------------------------
+// eprecond1 (fwi/preconditioning.py:37-62): We = sqrt(Ws Wr) + eps max(...)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_sqrt_prod(const T *__restrict__ a, const T *__restrict__ b, long long n, T *__restrict__ out)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        out[i] = sqrt(a[i] * b[i]);
+}
+
+// part[blockIdx.x] = max (op = 1) or min (op = 0) of a grid-stride share of x
+template <typename T>
+__global__ __launch_bounds__(256) void k_minmax_part(const T *__restrict__ x, long long n, int op, T *__restrict__ part)
+{
+    __shared__ T red[256];
+    T m = x[0];
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        m = op ? (x[i] > m ? x[i] : m) : (x[i] < m ? x[i] : m);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            const T o = red[threadIdx.x + h];
+            red[threadIdx.x] = op ? (o > red[threadIdx.x] ? o : red[threadIdx.x]) : (o < red[threadIdx.x] ? o : red[threadIdx.x]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// x += (T)(eps * max), max = the single value at *mx
+template <typename T>
+__global__ void k_add_eps_max(T *__restrict__ x, long long n, const T *__restrict__ mx, double eps)
+{
+    const T add = (T)(eps * (double)*mx);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        x[i] = x[i] + add;
+}
+
+// ---------------------------------------------------------------------------
+// taper_grad (fwi/preconditioning.py:144-183): depth taper, broadcast along x
+// ---------------------------------------------------------------------------
+__global__ void k_taper_grad(int nz, int nx, double dz, int gradt1, int gradt2, double exp_taper, int bottom, int gradb1,
+                             int gradb2, double *__restrict__ out)
+{
+    const long long n = (long long)nz * nx;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+        const int j = (int)(e / nx);
+        double t;
+        if (j < gradt1) t = 0.0;
+        else if (j <= gradt2) t = exp(-((double)(gradt2 - j) * (double)(gradt2 - j)));
+        else t = pow(j * dz, exp_taper);
+        if (bottom) {
+            const double den = fmax((double)(gradb2 - gradb1), 1.0);
+            if (j > gradb2) t = 0.0;
+            else if (j >= gradb1) t *= exp(-((double)(j - gradb1) * (double)(j - gradb1)) / (den * den) * 4.0);
+        }
+        out[e] = t;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// build_source_taper (fwi/taper.py:16-84)
+// ---------------------------------------------------------------------------
+// the normalised erf patch (ps x ps), one workgroup: erf(r / (sqrt 2 R)),
+// minus its minimum, divided by the largest of its four edge-centre values,
+// clipped to [0, 1]
+__global__ __launch_bounds__(256) void k_src_patch(int ps, double dh, double srt_radius, double *__restrict__ patch)
+{
+    __shared__ double red[256];
+    const double center = ps / 2.0, maxrad = sqrt(2.0) * srt_radius;
+    double m = 1e300;
+    for (int e = threadIdx.x; e < ps * ps; e += blockDim.x) {
+        const int jj = e / ps, ii = e - jj * ps;
+        const double x = (ii - center + 0.5) * dh, y = (jj - center + 0.5) * dh;
+        const double v = erf(sqrt(x * x + y * y) / maxrad);
+        patch[e] = v;
+        m = fmin(m, v);
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + h]);
+        __syncthreads();
+    }
+    const double pmin = red[0];
+    __syncthreads();
+    for (int e = threadIdx.x; e < ps * ps; e += blockDim.x) patch[e] -= pmin;
+    __syncthreads();
+    const int mid = ps / 2;
+    const double ev = fmax(fmax(patch[mid], patch[mid * ps]), fmax(patch[mid * ps + ps - 1], patch[(ps - 1) * ps + mid]));
+    __syncthreads();
+    for (int e = threadIdx.x; e < ps * ps; e += blockDim.x) {
+        const double v = patch[e] / ev;
+        patch[e] = v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
+    }
+}
+
+// taper[gz][gx] = min(1, min over sources whose patch covers the cell)
+__global__ void k_src_stamp(int nz, int nx, int ps, const double *__restrict__ patch, const int *__restrict__ src,
+                            int nsrc, double *__restrict__ out)
+{
+    const long long n = (long long)nz * nx;
+    const int ijc = ps / 2;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+        const int gz = (int)(e / nx), gx = (int)(e - (long long)gz * nx);
+        double t = 1.0;
+        for (int s = 0; s < nsrc; ++s) {
+            const int pi = gx - src[2 * s] + ijc, pj = gz - src[2 * s + 1] + ijc;
+            if (pi >= 0 && pi < ps && pj >= 0 && pj < ps) t = fmin(t, patch[pj * ps + pi]);
+        }
+        out[e] = t;
+    }
+}
+
+// taper = (taper - min) / max(taper - min) when that max is > 0, then zero
+// the (2 filt + 1)^2 box around every source; mm = {min, max} of the input
+__global__ void k_src_finish(int nz, int nx, const double *__restrict__ mm, const int *__restrict__ src, int nsrc,
+                             int filt, double *__restrict__ out)
+{
+    const long long n = (long long)nz * nx;
+    const double lo = mm[0], hi = mm[1] - mm[0];
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+        const int gz = (int)(e / nx), gx = (int)(e - (long long)gz * nx);
+        double t = out[e] - lo;
+        if (hi > 0) t = t / hi;
+        for (int s = 0; s < nsrc; ++s)
+            if (gz >= src[2 * s + 1] - filt && gz <= src[2 * s + 1] + filt && gx >= src[2 * s] - filt &&
+                gx <= src[2 * s] + filt)
+                t = 0.0;
+        out[e] = t;
+    }
+}
+
 }  // namespace stfwi

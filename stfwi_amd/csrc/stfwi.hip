@@ -1080,6 +1080,89 @@ extern "C" int stfwi_eprecond3(const void *ws, int n_shots, int nz_t, int nx_t, 
     return 0;
 }
 
+// blocks of 256 threads for a grid-stride loop over n elements
+static int grid1(long long n) { return (int)std::min<long long>((n + 255) / 256, 8192); }
+
+// min (op = 0) or max (op = 1) of x[0, n) into *res: two passes through a
+// pool-allocated partials buffer
+template <typename T>
+static int dev_minmax(const T *x, long long n, int op, T *res, hipStream_t st)
+{
+    const int nb = (int)std::min<long long>((n + 255) / 256, 1024);
+    T *part = nullptr;
+    HIP_TRY(hipMallocAsync((void **)&part, nb * sizeof(T), st));
+    hipLaunchKernelGGL(k_minmax_part<T>, dim3(nb), dim3(256), 0, st, x, n, op, part);
+    hipLaunchKernelGGL(k_minmax_part<T>, dim3(1), dim3(256), 0, st, (const T *)part, (long long)nb, op, res);
+    HIP_TRY(hipFreeAsync(part, st));
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+template <typename T>
+static int eprecond1_t(const T *ws, const T *wr, long long n, double eps, T *we, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sqrt_prod<T>, dim3(grid1(n)), dim3(256), 0, st, ws, wr, n, we);
+    T *mx = nullptr;
+    HIP_TRY(hipMallocAsync((void **)&mx, sizeof(T), st));
+    const int rc = dev_minmax<T>(we, n, 1, mx, st);
+    if (rc == 0) hipLaunchKernelGGL(k_add_eps_max<T>, dim3(grid1(n)), dim3(256), 0, st, we, n, (const T *)mx, eps);
+    HIP_TRY(hipFreeAsync(mx, st));
+    HIP_TRY(hipGetLastError());
+    return rc;
+}
+
+extern "C" int stfwi_eprecond1(const void *ws, const void *wr, long long n, double eps, int dtype, void *we, void *stream)
+{
+    if (!ws || !wr || !we || n < 1) return fail(STFWI_ERR_INVALID, "bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    if (dtype == STFWI_F64) return eprecond1_t<double>((const double *)ws, (const double *)wr, n, eps, (double *)we, st);
+    if (dtype == STFWI_F32) return eprecond1_t<float>((const float *)ws, (const float *)wr, n, eps, (float *)we, st);
+    return fail(STFWI_ERR_INVALID, "dtype=%d", dtype);
+}
+
+extern "C" int stfwi_taper_grad(int nz, int nx, double dz, int gradt1, int gradt2, double exp_taper, int bottom,
+                                int gradb1, int gradb2, double *out, void *stream)
+{
+    if (!out || nz < 1 || nx < 1) return fail(STFWI_ERR_INVALID, "bad argument");
+    const long long n = (long long)nz * nx;
+    hipLaunchKernelGGL(k_taper_grad, dim3(grid1(n)), dim3(256), 0, (hipStream_t)stream, nz, nx, dz, gradt1, gradt2,
+                       exp_taper, bottom, gradb1, gradb2, out);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int stfwi_source_taper(int nz, int nx, double dh, const int *src_xz, int nsrc, double srt_radius,
+                                  int filt_size, double *out, void *stream)
+{
+    if (!out || nz < 1 || nx < 1 || nsrc < 0 || (nsrc > 0 && !src_xz) || !(dh > 0) || !(srt_radius > 0))
+        return fail(STFWI_ERR_INVALID, "bad argument");
+    const int ps = std::max((int)std::ceil(2.0 * srt_radius / dh), 5);
+    if ((long long)ps * ps > (1LL << 26)) return fail(STFWI_ERR_INVALID, "taper radius too large for the spacing");
+    hipStream_t st = (hipStream_t)stream;
+    const long long n = (long long)nz * nx;
+    double *patch = nullptr, *mm = nullptr;
+    int *src = nullptr;
+    HIP_TRY(hipMallocAsync((void **)&patch, (size_t)ps * ps * sizeof(double), st));
+    HIP_TRY(hipMallocAsync((void **)&mm, 2 * sizeof(double), st));
+    HIP_TRY(hipMallocAsync((void **)&src, (size_t)std::max(nsrc, 1) * 2 * sizeof(int), st));
+    if (nsrc > 0) HIP_TRY(hipMemcpyAsync(src, src_xz, (size_t)nsrc * 2 * sizeof(int), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_src_patch, dim3(1), dim3(256), 0, st, ps, dh, srt_radius, patch);
+    hipLaunchKernelGGL(k_src_stamp, dim3(grid1(n)), dim3(256), 0, st, nz, nx, ps, (const double *)patch,
+                       (const int *)src, nsrc, out);
+    int rc = dev_minmax<double>(out, n, 0, mm, st);
+    if (rc == 0) rc = dev_minmax<double>(out, n, 1, mm + 1, st);
+    if (rc == 0)
+        hipLaunchKernelGGL(k_src_finish, dim3(grid1(n)), dim3(256), 0, st, nz, nx, (const double *)mm,
+                           (const int *)src, nsrc, filt_size, out);
+    HIP_TRY(hipFreeAsync(patch, st));
+    HIP_TRY(hipFreeAsync(mm, st));
+    HIP_TRY(hipFreeAsync(src, st));
+    HIP_TRY(hipGetLastError());
+    // the host source list is read by the copy above before this returns
+    HIP_TRY(hipStreamSynchronize(st));
+    return rc;
+}
+
 template <typename T>
 static int sos_t(void *x, int nt, int ntr, const double *sos, int n_sos, int n_taper, int transpose, hipStream_t st)
 {
@@ -1199,7 +1282,6 @@ extern "C" int stfwi_set_graphs(stfwi_plan *p, int enable)
 // ---------------------------------------------------------------------------
 // learned preconditioner: BF16-MFMA convolutions and FP32 layer kernels
 // ---------------------------------------------------------------------------
-static int grid1(long long n) { return (int)std::min<long long>((n + 255) / 256, 8192); }
 
 // the LDS-tiled convolution (input channels of the call <= 64): the weights
 // are packed once per call into a stream-ordered pool allocation, then every

@@ -1,7 +1,9 @@
-"""Drop-in for the preconditioner on the path (fwi/preconditioning.py:65-139)
-and its driver (fwi/optim_lbfgsb.py:46-100), on the HIP device.
+"""Drop-ins for fwi/preconditioning.py and the eprecond3 driver
+(fwi/optim_lbfgsb.py:46-100), on the HIP device.
 
-    eprecond3(Ws, rec_x, rec_z, dx, dz, free_surface, pad, eps=5e-7) -> We
+    eprecond1(Ws, Wr, eps=5e-7) -> We                       preconditioning.py:37-62
+    eprecond3(Ws, rec_x, rec_z, dx, dz, free_surface, pad, eps=5e-7) -> We   :65-139
+    taper_grad(shape, dz, gradt1, gradt2, exp_taper=2.0, gradb1=None, gradb2=None)   :144-183
     eprecond3_preconditioner(...) -> normalised physical multiplier
 """
 from __future__ import annotations
@@ -49,3 +51,34 @@ def eprecond3_preconditioner(*, nz_dom, nx_dom, dx, dz, dt, nt, f_c, pad, rec_x,
     ws = shot_energy(nz_dom, nx_dom, dx, dz, dt, nt, f_c, pad, rec_x, rec_z, fd_order, component, Vs, Vp, rho,
                      src_x, src_z, src_wavelet, free_surface=free_surface, src_type=src_type)
     return _call(ws, rec_x, rec_z, dx, dz, free_surface, pad, int(nz_dom), int(nx_dom), eps)[1]
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def eprecond1(Ws, Wr, eps=5e-7):
+    """Shin et al. energy weight We = sqrt(Ws Wr) + eps max(sqrt(Ws Wr)), in the
+    promoted floating dtype of Ws and Wr (fwi/preconditioning.py:37-62)."""
+    from ._common import storage_dtype
+    dt = torch.float64 if torch.float64 in (storage_dtype(Ws), storage_dtype(Wr)) else torch.float32
+    ws, wr = as_dev(Ws, dt), as_dev(Wr, dt)
+    if ws.shape != wr.shape:
+        raise ValueError(f"Ws {tuple(ws.shape)} and Wr {tuple(wr.shape)} differ in shape")
+    we = torch.empty_like(ws)
+    code = _abi.F64 if dt == torch.float64 else _abi.F32
+    _abi.check(_abi.lib().stfwi_eprecond1(C.c_void_p(ws.data_ptr()), C.c_void_p(wr.data_ptr()), ws.numel(),
+                                          float(eps), code, C.c_void_p(we.data_ptr()), _stream()))
+    return we
+
+
+def taper_grad(shape, dz, gradt1, gradt2, exp_taper=2.0, gradb1=None, gradb2=None):
+    """DENISE depth taper (nz, nx), float64 (fwi/preconditioning.py:144-183)."""
+    from ._common import device
+    nz, nx = (int(v) for v in shape)
+    out = torch.empty((nz, nx), dtype=torch.float64, device=device())
+    bottom = gradb1 is not None and gradb2 is not None
+    _abi.check(_abi.lib().stfwi_taper_grad(nz, nx, float(dz), int(gradt1), int(gradt2), float(exp_taper),
+                                           int(bottom), int(gradb1) if bottom else 0, int(gradb2) if bottom else 0,
+                                           C.c_void_p(out.data_ptr()), _stream()))
+    return out

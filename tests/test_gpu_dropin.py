@@ -152,3 +152,28 @@ def test_default_precision_is_float32():
     rec = forward_jax(m["Vs"], m["Vp"], m["rho"], sx, sz, rx, rz, m["nx"], m["nz"], m["dx"], m["dz"], m["dt"],
                       m["nt"], m["fc"], m["pad"], s, fd_order=8)
     assert rec[0].dtype == torch.float32 and rec[0].shape == (m["nt"], rx.size)
+
+
+def test_eprecond1_taper_grad_and_source_taper_match_oracle():
+    """eprecond1 (fwi/preconditioning.py:37-62) in both precisions, the depth
+    taper (:144-183) with and without the bottom roll-off, and the source
+    taper (fwi/taper.py:16-84) for sources inside, at the edge of and outside
+    the model, against the oracle restatements."""
+    from stfwi_amd.fwi import eprecond1, taper_grad, build_source_taper
+    rng = np.random.default_rng(8)
+    ws = rng.random((44, 61)) ** 2 + 1e-6
+    wr = rng.random((44, 61)) ** 2 + 1e-6
+    we = eprecond1(ws, wr, eps=5e-7)
+    assert we.dtype == torch.float64
+    assert rel_l2(_np(we), opre.eprecond1(ws, wr, 5e-7)) < 1e-15
+    we32 = eprecond1(ws.astype(np.float32), wr.astype(np.float32), eps=5e-7)
+    assert we32.dtype == torch.float32
+    ref32 = opre.eprecond1(ws.astype(np.float32), wr.astype(np.float32), 5e-7)
+    assert ref32.dtype == np.float32 and rel_l2(_np(we32), ref32) < 1e-7
+    for kw in (dict(gradt1=4, gradt2=8, exp_taper=2.0), dict(gradt1=0, gradt2=3, exp_taper=1.5, gradb1=30, gradb2=40)):
+        t = taper_grad((44, 61), 7.5, **kw)
+        assert rel_l2(_np(t), opre.taper_grad((44, 61), 7.5, **kw)) < 1e-15
+    for srcs, r, f in (([(30, 2), (5, 40)], 45.0, 2), ([(0, 0), (60, 43), (80, 10)], 30.0, 1), ([], 20.0, 2)):
+        t = build_source_taper(44, 61, 10.0, srcs, r, filt_size=f)
+        ref = opre.source_taper(44, 61, 10.0, srcs, r, filt_size=f)
+        assert np.max(np.abs(_np(t) - ref)) < 1e-14, srcs

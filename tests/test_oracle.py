@@ -220,3 +220,58 @@ def test_oracle_matches_reference_gradient_compare():
     assert rel_l2(g_ad, fx["g_Vs_AD"]) < 1e-7
     # the Zhou and exact gradients differ (the reference's 0.73)
     assert 0.6 < rel_l2(g_hc, fx["g_Vs_AD"]) < 0.85
+
+
+def test_oracle_eprecond1_closed_forms_and_reference_checks():
+    """eprecond1 (fwi/preconditioning.py:37-62): equal energies reduce to
+    Ws + eps max Ws; the reference's own checks (test_phase1_modules.py:61-68):
+    shape kept, all entries positive."""
+    from oracle import precond as op
+    rng = np.random.default_rng(3)
+    ws = rng.random((30, 44)) ** 2 + 1e-6
+    wr = rng.random((30, 44)) ** 2 + 1e-6
+    we = op.eprecond1(ws, wr, eps=5e-7)
+    assert we.shape == ws.shape and np.all(we > 0)
+    same = op.eprecond1(ws, ws, eps=1e-3)
+    assert np.allclose(same, ws + 1e-3 * ws.max(), rtol=1e-15, atol=0)
+    i, j = 7, 19
+    assert we[i, j] == np.sqrt(ws[i, j] * wr[i, j]) + 5e-7 * np.sqrt(ws * wr).max()
+
+
+def test_oracle_taper_grad_values():
+    """taper_grad rows: 0 above gradt1, exp(-(gradt2-j)^2) up to gradt2 (1 at
+    gradt2), (j dz)^p below; the reference's checks
+    (test_phase1_modules.py:79-82); the bottom roll-off ends at 0."""
+    from oracle import precond as op
+    t = op.taper_grad((20, 7), dz=5.0, gradt1=4, gradt2=8, exp_taper=2.0)
+    assert np.all(t[:4] == 0.0) and np.all(t[8] > t[7])
+    assert np.all(t[8] == 1.0) and np.allclose(t[6], np.exp(-4.0), rtol=1e-15)
+    assert np.allclose(t[12], (12 * 5.0) ** 2, rtol=1e-15)
+    assert np.all(t == t[:, :1])
+    b = op.taper_grad((20, 3), dz=5.0, gradt1=2, gradt2=3, exp_taper=1.0, gradb1=10, gradb2=14)
+    assert np.all(b[15:] == 0.0) and np.allclose(b[10], 50.0) and np.allclose(b[12], 60.0 * np.exp(-1.0))
+
+
+def test_oracle_source_taper_against_scipy_erf_and_geometry():
+    """The erf patch against scipy.special.erf; the taper is 0 in the box at a
+    source, 1 far from every source, mirror-symmetric about an isolated
+    source and non-decreasing along a ray from it."""
+    from scipy.special import erf
+    from oracle import precond as op
+    dh, R = 10.0, 45.0
+    p = op.source_patch(dh, R)
+    ps = p.shape[0]
+    c = ps / 2.0
+    jj, ii = np.mgrid[0:ps, 0:ps]
+    q = erf(np.hypot((ii - c + 0.5) * dh, (jj - c + 0.5) * dh) / (np.sqrt(2.0) * R))
+    q -= q.min()
+    m = ps // 2
+    q = np.clip(q / max(q[0, m], q[m, 0], q[m, -1], q[-1, m]), 0, 1)
+    assert np.allclose(p, q, rtol=0, atol=1e-15)
+    t = op.source_taper(40, 60, dh, [(30, 20)], R, filt_size=2)
+    assert np.all(t[18:23, 28:33] == 0.0) and t[0, 0] == 1.0 and t.min() == 0.0 and t.max() == 1.0
+    assert np.allclose(t[20, 30 - 5:30], t[20, 31:36][::-1]) or ps % 2 == 0
+    row = t[20, 30:]
+    assert np.all(np.diff(row) >= -1e-15)
+    two = op.source_taper(40, 60, dh, [(10, 5), (50, 30)], R, filt_size=1)
+    assert two[5, 10] == 0.0 and two[30, 50] == 0.0 and np.all((two >= 0) & (two <= 1))
