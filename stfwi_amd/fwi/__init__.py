@@ -11,3 +11,4 @@ from .loss import build_loss_fn, gaussian_smooth_2d  # noqa: F401
 from .filters import jax_biquad, jax_sosfilt, cosine_taper_end, sosfilt_T  # noqa: F401
 from .projections import project_velocity, cfl_vp_cap, k_safe_ratio  # noqa: F401
 from .optim_lbfgsb import run_multiscale_fwi  # noqa: F401
+from .optimizer import run_lbfgs, run_lbfgs_alternating  # noqa: F401

@@ -201,7 +201,8 @@ def _armijo_projected(ev: _Eval, f0, g0, x0, t0, c1, maxls):
 def minimize(fun: Callable, x0: torch.Tensor, lo: Optional[torch.Tensor] = None, hi: Optional[torch.Tensor] = None,
              maxiter: int = 50, m: int = 10, maxls: int = 20, c1: float = 1e-4, c2: float = 0.9,
              callback: Optional[Callable] = None, memory: Optional[LBFGSMemory] = None,
-             direction_fn: Optional[Callable] = None, maxls_learned: int = 10, on_step: Optional[Callable] = None):
+             direction_fn: Optional[Callable] = None, maxls_learned: int = 10, on_step: Optional[Callable] = None,
+             max_step: Optional[float] = None, fg0: Optional[tuple] = None, t_init: Optional[float] = None):
     """Minimise fun(x) -> (f, g) over float64 device vectors x, with optional
     per-entry bounds lo <= x <= hi (device tensors or None).
 
@@ -215,6 +216,11 @@ def minimize(fun: Callable, x0: torch.Tensor, lo: Optional[torch.Tensor] = None,
     decrease of f, the plain L-BFGS step is taken instead.  on_step(x_old,
     g_old, x_new, g_new, learned) sees every accepted step.
 
+    max_step caps the step length t of the line search (jaxopt's
+    max_stepsize); fg0 = (f, g) at x0 saves the first evaluation when the
+    caller already has it; t_init is the first trial step while the memory
+    holds no pair (default min(1, 1/|d|)).
+
     Returns a dict: x, fun, jac, nit, nfev, success, message, n_learned
     (accepted learned steps), n_fallback (learned proposals rejected)."""
     if x0.dtype != torch.float64 or not x0.is_cuda:
@@ -223,9 +229,15 @@ def minimize(fun: Callable, x0: torch.Tensor, lo: Optional[torch.Tensor] = None,
     if lo is not None or hi is not None:
         x = step_clip(x, torch.zeros_like(x), 0.0, lo, hi)
     mem = memory or LBFGSMemory(x.numel(), m, x.device)
-    f, g = fun(x)
+    if fg0 is not None:
+        f, g = fg0
+        nfev = 0
+    else:
+        f, g = fun(x)
+        nfev = 1
     f = float(f)
-    nfev, nit = 1, 0
+    nit = 0
+    t_cap = float(max_step) if max_step is not None else 1e20
     d = torch.empty_like(x)
     msg, ok = "maximum number of iterations reached", True
     bounded = lo is not None or hi is not None
@@ -253,7 +265,8 @@ def minimize(fun: Callable, x0: torch.Tensor, lo: Optional[torch.Tensor] = None,
         t0 = 1.0
         if nit == 0 and len(mem) == 0:
             dn = math.sqrt(dot(d, d))
-            t0 = min(1.0, 1.0 / dn) if dn > 0 else 1.0
+            t0 = t_init if t_init is not None else (min(1.0, 1.0 / dn) if dn > 0 else 1.0)
+        t0 = min(t0, t_cap)
         res, learned = None, False
         if direction_fn is not None:
             dl = direction_fn(x, g)
@@ -276,7 +289,7 @@ def minimize(fun: Callable, x0: torch.Tensor, lo: Optional[torch.Tensor] = None,
         if res is not None:
             pass
         elif lo is None and hi is None:
-            res = _wolfe(ev, f, gd, t0, c1, c2, maxls)
+            res = _wolfe(ev, f, gd, t0, c1, c2, maxls, t_max=t_cap)
         else:
             res = _armijo_projected(ev, f, g, x, t0, c1, maxls)
         nfev += ev.nfev

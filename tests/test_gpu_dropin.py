@@ -177,3 +177,47 @@ def test_eprecond1_taper_grad_and_source_taper_match_oracle():
         t = build_source_taper(44, 61, 10.0, srcs, r, filt_size=f)
         ref = opre.source_taper(44, 61, 10.0, srcs, r, filt_size=f)
         assert np.max(np.abs(_np(t) - ref)) < 1e-14, srcs
+
+
+def _tiny_inversion(tmp_path):
+    """Observed vz from a Vs/Vp anomaly on the 24x32 model, four force shots,
+    and the drop-in loss_fn (build_loss_fn) on the smooth starting model."""
+    from stfwi_amd.fwi import build_forward_fn, build_loss_fn, ricker_jax
+    m = dp_model()
+    nt = 160
+    rx = np.arange(2, m["nx"] - 2, 2)
+    rz = np.zeros_like(rx)
+    run = build_forward_fn(m["nz"], m["nx"], m["dx"], m["dz"], m["dt"], nt, m["fc"], m["pad"], nt, rx, rz,
+                           fd_order=4, free_surface=True)
+    wav = _np(ricker_jax(np.arange(nt) * m["dt"], m["fc"], 1.5 / m["fc"])) * 1e6
+    sx = np.array([4, 12, 20, 28])
+    zz, xx = np.meshgrid(np.arange(m["nz"]), np.arange(m["nx"]), indexing="ij")
+    bump = np.exp(-((zz - 12) ** 2 + (xx - 16) ** 2) / 18.0)
+    vs_t, vp_t = m["Vs"] * (1 + 0.05 * bump), m["Vp"] * (1 + 0.03 * bump)
+    obs = np.stack([_np(run(vs_t, vp_t, m["rho"], int(s), 2, wav)[1]) for s in sx])
+    scale = 1.0 / np.mean(obs ** 2)
+    return m, sx, wav, obs, scale, run
+
+
+def test_run_lbfgs_dropins_decrease_misfit_and_save_models(tmp_path, x64):
+    """run_lbfgs (fwi/optimizer.py:20-168) over Vs only and over all three
+    parameters, and run_lbfgs_alternating (:171-284): the misfit decreases,
+    frozen parameters stay put, and the inverted models and loss history are
+    written as the reference names them."""
+    from stfwi_amd.fwi import build_loss_fn, run_lbfgs, run_lbfgs_alternating
+    m, sx, wav, obs, scale, run = _tiny_inversion(tmp_path)
+    init = (m["Vs"].copy(), m["Vp"].copy(), m["rho"].copy())
+    lf = build_loss_fn(run, obs, sx, np.full(sx.size, 2), wav, component=1, scale=scale, invert="vs")
+    p, hist = run_lbfgs(lf, init, 4, save_dir=str(tmp_path / "a"), invert="vs")
+    assert len(hist) == 4 and hist[-1] < hist[0]
+    assert not np.array_equal(p[0], init[0]) and np.array_equal(p[1], init[1]) and np.array_equal(p[2], init[2])
+    for n in ("Vs_inverted.npy", "Vp_inverted.npy", "rho_inverted.npy", "loss_history.npy"):
+        assert (tmp_path / "a" / n).exists()
+    assert np.array_equal(np.load(tmp_path / "a" / "Vs_inverted.npy"), p[0])
+    lf2 = build_loss_fn(run, obs, sx, np.full(sx.size, 2), wav, component=1, scale=scale, invert="vs_vp")
+    p2, hist2 = run_lbfgs_alternating(lf2, init, 3, save_dir=str(tmp_path / "b"), invert="vs_vp")
+    assert hist2[-1] < hist2[0]
+    assert not np.array_equal(p2[0], init[0]) and not np.array_equal(p2[1], init[1])
+    assert np.array_equal(p2[2], init[2])
+    p3, hist3 = run_lbfgs(lf2, init, 2, save_dir=str(tmp_path / "c"), invert="vs_vp", early_stop_ratio=2.0)
+    assert len(hist3) == 1        # ratio 1 < 2 stops after the first epoch
