@@ -234,6 +234,13 @@ int stfwi_set_tiles(stfwi_plan *plan, const int *tz);
  * to the eager launches; enable = 0 (the default) launches eagerly. */
 int stfwi_set_graphs(stfwi_plan *plan, int enable);
 
+/* Adjoint wavefield energy (no reference counterpart; the Wr that eprecond1,
+ * fwi/preconditioning.py:37-62, takes): while energy is non-NULL every
+ * stfwi_adjoint adds, at every reverse step, the square of adjoint field comp
+ * (0 U_a, 1 V_a, 2 exx_a, 3 ezz_a, 4 exz_a; the values snapshots record) to
+ * energy[shot][nz_t][nx_t] on the padded grid.  The caller zeroes it. */
+int stfwi_set_adjoint_energy(stfwi_plan *plan, void *energy, int comp);
+
 /* Split the shots into `groups` (1..4) contiguous groups whose time loops
  * run concurrently on the plan's own streams (default priority), joined
  * back into the caller's stream; results do not depend on the grouping. */

@@ -42,6 +42,7 @@ SIGNATURES = {
     "stfwi_eprecond3": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _VP, _VP,
                                   C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _VP, _VP, _VP]),
     "stfwi_eprecond1": (C.c_int, [_VP, _VP, C.c_longlong, C.c_double, C.c_int, _VP, _VP]),
+    "stfwi_set_adjoint_energy": (C.c_int, [_VP, _VP, C.c_int]),
     "stfwi_taper_grad": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
                                    C.c_int, _VP, _VP]),
     "stfwi_source_taper": (C.c_int, [C.c_int, C.c_int, C.c_double, _VP, C.c_int, C.c_double, C.c_int, _VP, _VP]),

@@ -808,6 +808,8 @@ struct AdjArgs {
     T *gacc;                           // [shot][nacc][np]
     T *snap;
     int snap_mask, nsnap;
+    T *aen;                            // [shot][nz*nx]: sum over steps of the squared adjoint field aen_comp
+    int aen_comp;
     int tiles_x, tiles_z, ntiles;
     int shot0;                         // first shot of this launch's group
 };
@@ -1543,14 +1545,22 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
                 }
             }
 
-            if (a.snap) {
+            if (a.snap || a.aen) {
                 const T vals[5] = {Ua[q], Va[q], exan, ezan, esan};
-                int k = 0;
-                for (int f = 0; f < 5; ++f)
-                    if (a.snap_mask & (1 << f)) {
-                        a.snap[(((i64)s * a.nsnap + k) * a.nt_max + it) * nz * nx + (i64)r * nx + c] = vals[f];
-                        ++k;
-                    }
+                if (a.snap) {
+                    int k = 0;
+                    for (int f = 0; f < 5; ++f)
+                        if (a.snap_mask & (1 << f)) {
+                            a.snap[(((i64)s * a.nsnap + k) * a.nt_max + it) * nz * nx + (i64)r * nx + c] = vals[f];
+                            ++k;
+                        }
+                }
+                if (a.aen) {
+                    // adjoint energy of one component (the Wr of eprecond1)
+                    const T v = vals[a.aen_comp];
+                    T *e = a.aen + (i64)s * nz * nx + (i64)r * nx + c;
+                    *e = *e + v * v;
+                }
             }
 
             // imaging on the physical cells

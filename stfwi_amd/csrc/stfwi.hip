@@ -91,6 +91,9 @@ struct stfwi_plan {
     // counts of merged receiver cells, and the per-step merged sums
     int *cellmap_d = nullptr, *rowcum_d = nullptr;
     double *lpart = nullptr;   // [shot][nrec_max] receiver sums of squares of the residual
+    // adjoint energy accumulator of the next stfwi_adjoint (caller-owned)
+    void *aen = nullptr;
+    int aen_comp = 0;
     void *injm = nullptr;
     std::vector<int> shot_nt;
     i64 bytes = 0;
@@ -541,6 +544,7 @@ static AdjArgs<T> adj_args(stfwi_plan *p, const void *rec_adj, void *src_adj, vo
     if (p->ck) a.state = (T *)p->state_a;
     a.gacc = (T *)p->gacc;
     a.snap = (T *)snap; a.snap_mask = snap_mask; a.nsnap = popcount5(snap_mask);
+    a.aen = (T *)p->aen; a.aen_comp = p->aen_comp;
     a.tiles_x = (p->nx + TILE_X - 1) / TILE_X;
     return a;
 }
@@ -801,7 +805,7 @@ extern "C" int stfwi_adjoint(stfwi_plan *p, const void *rec_adj, void *src_adj, 
                        : adjoint_t<float, 2>(p, ra, sa, sn, sm, s2);
     };
     int rc;
-    if (p->graphs && !snap) {
+    if (p->graphs && !snap && !p->aen) {
         const size_t rbytes = (size_t)p->n_shots * 4 * p->cfg.nt_max * std::max(p->cfg.nrec_max, 1) * p->esz;
         HIP_TRY(hipMemcpyAsync(p->radj_int, rec_adj, rbytes, hipMemcpyDeviceToDevice, st));
         const int want = src_adj ? 1 : 0;
@@ -977,6 +981,14 @@ extern "C" int stfwi_autotune(stfwi_plan *p, int reps, void *stream, int *tz_out
     // the timing launches ran on the live buffers
     p->fwd_done = false;
     p->adj_done = false;
+    return 0;
+}
+
+extern "C" int stfwi_set_adjoint_energy(stfwi_plan *p, void *energy, int comp)
+{
+    if (!p || comp < 0 || comp > 4) return fail(STFWI_ERR_INVALID, "bad argument (comp=%d)", comp);
+    p->aen = energy;
+    p->aen_comp = comp;
     return 0;
 }
 

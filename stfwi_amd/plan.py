@@ -194,7 +194,12 @@ class Plan:
                                    float(scale), int(n_norm or S), _ptr(loss), _ptr(radj), _stream()))
         return loss, radj
 
-    def adjoint(self, rec_adj, want_src=True, snap=()):
+    def adjoint(self, rec_adj, want_src=True, snap=(), energy=None):
+        """Reverse time loop for receiver adjoints rec_adj (n_shots, 4,
+        nt_max, nrec_max).  Returns 'src_adj' (n_shots, nt_max), optional
+        'snap' as forward(), and with energy = 'vx' | 'vz' | 'ex' | 'ez' | 'es'
+        'energy' (n_shots, nz_t, nx_t): the sum over steps of that adjoint
+        field squared (the Wr of eprecond1)."""
         S = self.n_shots
         ra = self._t(rec_adj)
         if tuple(ra.shape) != (S, 4, self.nt_max, self.nrec_max):
@@ -213,7 +218,19 @@ class Plan:
             out["snap"] = sn
             out["snap_order"] = order
         self._keep["radj"] = ra
-        check(lib().stfwi_adjoint(self._h, _ptr(ra), _ptr(sa), _ptr(sn), smask, _stream()))
+        en = None
+        if energy is not None:
+            comps = ("vx", "vz", "ex", "ez", "es")
+            if energy not in comps:
+                raise ValueError(f"energy={energy!r} (use one of {comps})")
+            en = torch.zeros((S,) + self.shape, dtype=self.dtype, device=self.device)
+            check(lib().stfwi_set_adjoint_energy(self._h, _ptr(en), comps.index(energy)))
+            out["energy"] = en
+        try:
+            check(lib().stfwi_adjoint(self._h, _ptr(ra), _ptr(sa), _ptr(sn), smask, _stream()))
+        finally:
+            if en is not None:
+                check(lib().stfwi_set_adjoint_energy(self._h, None, 0))
         return out
 
     def autotune(self, reps=20):

@@ -293,3 +293,32 @@ def test_gpu_stream_groups_agree_bitwise():
     for o in outs[1:]:
         for x, y in zip(o, outs[0]):
             assert torch.equal(x, y)
+
+
+@pytest.mark.parametrize("comp", ["vx", "ez"])
+def test_gpu_adjoint_energy_equals_sum_of_squared_snapshots(comp):
+    """The adjoint energy accumulator (the Wr of eprecond1,
+    fwi/preconditioning.py:37-62) equals the sum over reverse steps of the
+    squared adjoint snapshots of the same component, per shot, with stream
+    groups; the next adjoint without it leaves the buffer alone."""
+    m = dp_model()
+    fs, st = True, "moment"
+    sx, sz, rx, rz = dp_geometry(m["nz"], m["nx"], fs, st)
+    rng = np.random.default_rng(21)
+    p = _plan(m, 8, fs, torch.float64, nshots=2, nrec=rx.size)
+    p.set_model(m["Vs"], m["Vp"], m["rho"])
+    p.set_shots([sx, sx + 5], [sz, sz], st, [m["nt"], m["nt"] - 13], rx, rz, np.ones(m["nt"]))
+    p.set_groups(2)
+    ra = torch.as_tensor(rng.standard_normal((2, 4, m["nt"], rx.size)))
+    ra[1, :, m["nt"] - 13:] = 0
+    out = p.adjoint(ra, snap=(comp,), energy=comp)
+    e = out["energy"].cpu().numpy()
+    sn = out["snap"][:, 0].cpu().numpy()
+    ref = np.zeros_like(e)
+    for s, nt in enumerate([m["nt"], m["nt"] - 13]):
+        for t in range(nt - 1, -1, -1):
+            ref[s] += sn[s, t] ** 2
+    assert rel_l2(e, ref) < 1e-14
+    assert np.all(e >= 0) and np.any(e > 0)
+    again = p.adjoint(ra)
+    assert "energy" not in again
