@@ -809,7 +809,10 @@ extern "C" int stfwi_adjoint(stfwi_plan *p, const void *rec_adj, void *src_adj, 
         const size_t rbytes = (size_t)p->n_shots * 4 * p->cfg.nt_max * std::max(p->cfg.nrec_max, 1) * p->esz;
         HIP_TRY(hipMemcpyAsync(p->radj_int, rec_adj, rbytes, hipMemcpyDeviceToDevice, st));
         const int want = src_adj ? 1 : 0;
-        const std::vector<long long> key = {p->n_shots, p->nt_run, p->tz[2], p->tz[3], p->groups, want, p->max_nuc};
+        // the launch sequence depends on every shot's record length (the
+        // separate first-step injection), so all of them are in the key
+        std::vector<long long> key = {p->n_shots, p->nt_run, p->tz[2], p->tz[3], p->groups, want, p->max_nuc};
+        key.insert(key.end(), p->shot_nt.begin(), p->shot_nt.end());
         rc = run_graph(p, 1, key, st, [&](hipStream_t cs) {
             if (want) fill_rows(p->sadj_int, 1, 0, (i64)p->n_shots * p->cfg.nt_max, p->esz, cs);
             return dispatch(p->radj_int, want ? p->sadj_int : nullptr, nullptr, 0, cs);
