@@ -993,14 +993,18 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
     }
     // C-PML inputs of the staged cells and the owned cells' G values, kept
     // for the memory update after the barrier (tiles touching a strip only)
-    PmlIn<T> pz[KZ], px[KZ];
+    PmlIn<T> pz[KZ] = {}, px[KZ] = {};
     int izs[KZ], ixs[KZ];
     T gzu_o[RPT], gzv_o[RPT], gxv_o[RPT], gxu_o[RPT];
 #pragma unroll
     for (int k = 0; k < KZ; ++k) izs[k] = ixs[k] = -1;
 #pragma unroll
     for (int rr = 0; rr < RPT; ++rr) gzu_o[rr] = gzv_o[rr] = gxv_o[rr] = gxu_o[rr] = T(0);
-    if (inx && inz) {
+    // staging, compiled per tile class: along an interior axis the C-PML
+    // inputs and the edge masks of the staged cells are compiled out
+    auto stage = [&](auto tagx, auto tagz) {
+        constexpr bool IX = decltype(tagx)::value, IZ = decltype(tagz)::value;
+    if (IX && IZ) {
         // no PML, no masks: the adjoints are the scaled strain adjoints
 #pragma unroll
         for (int k = 0; k < KZ; ++k) {
@@ -1028,10 +1032,11 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
 #pragma unroll
         for (int k = 0; k < KZ; ++k) {
             const int r = r0 - 4 + ty + BLOCK_Y * k;
-            izs[k] = zok[k] ? zstrip(g, r, c) : -1;
-            ixs[k] = zok[k] ? xstrip(g, r, c) : -1;
-            pz[k].template load<true>(S.pzo, S.pzo + S.pzn, izs[k], a.profz, r, nz);
-            if (k >= 1 && k < TZ / BLOCK_Y + 1) px[k].template load<true>(S.pxo, S.pxo + S.pxn, ixs[k], a.profx, c, nx);
+            izs[k] = (!IZ && zok[k]) ? zstrip(g, r, c) : -1;
+            ixs[k] = (!IX && zok[k]) ? xstrip(g, r, c) : -1;
+            if (!IZ) pz[k].template load<true>(S.pzo, S.pzo + S.pzn, izs[k], a.profz, r, nz);
+            if (!IX && k >= 1 && k < TZ / BLOCK_Y + 1)
+                px[k].template load<true>(S.pxo, S.pxo + S.pxn, ixs[k], a.profx, c, nx);
         }
         PmlIn<T> hp[KX];
         int ihs[KX];
@@ -1039,15 +1044,15 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
         for (int k = 0; k < KX; ++k) {
             const int e = tid + 256 * k;
             const int zi = e >> 3, h = e & 7, xi = h < 4 ? h : h + TILE_X;
-            ihs[k] = hok[k] ? xstrip(g, r0 + zi, c0 - 4 + xi) : -1;
-            hp[k].template load<true>(S.pxo, S.pxo + S.pxn, ihs[k], a.profx, c0 - 4 + xi, nx);
+            ihs[k] = (!IX && hok[k]) ? xstrip(g, r0 + zi, c0 - 4 + xi) : -1;
+            if (!IX) hp[k].template load<true>(S.pxo, S.pxo + S.pxn, ihs[k], a.profx, c0 - 4 + xi, nx);
         }
         // Dx_U_a input: dt/dx exx_a (columns >= 1) plus, on the free-surface
         // row, the image term L_ratio dz/dx Dz_V_a (row 0 has no z-strip)
         auto gxu = [&](int r, int cc, T exv, T ezv) {
             T G = 0;
-            if (g.fs && r == 0 && cc <= nx - 2) G = a.Lr0[cc] * g.dzdx * (g.dtdz * ezv);
-            if (cc >= 1) G = G + g.dtdx * exv;
+            if (!IZ && g.fs && r == 0 && CX(cc <= nx - 2)) G = a.Lr0[cc] * g.dzdx * (g.dtdz * ezv);
+            if (CX(cc >= 1)) G = G + g.dtdx * exv;
             return G;
         };
 #pragma unroll
@@ -1055,7 +1060,7 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
             const int zi = ty + BLOCK_Y * k, r = r0 - 4 + zi;
             T u = 0, v = 0, xv = 0, xu = 0;
             if (zok[k]) {
-                const bool in2 = r <= nz - 2 && c <= nx - 2;
+                const bool in2 = CZ(r <= nz - 2) && CX(c <= nx - 2);
                 const T gzu = in2 ? g.hdtdz * zs[k] : T(0);
                 const T gzv = g.dtdz * zz[k];
                 u = izs[k] >= 0 ? pmlT_pre(gzu, pz[k].m0, pz[k].q, pz[k].a) : gzu;
@@ -1088,7 +1093,7 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
                 const int r = r0 + zi, cc = c0 - 4 + xi;
                 T xv = 0, xu = 0;
                 if (hok[k]) {
-                    const T gxv = (r <= nz - 2 && cc <= nx - 2) ? g.hdtdx * hs[k] : T(0);
+                    const T gxv = (CZ(r <= nz - 2) && CX(cc <= nx - 2)) ? g.hdtdx * hs[k] : T(0);
                     const T gxu_ = gxu(r, cc, hx[k], hz[k]);
                     xv = ihs[k] >= 0 ? pmlT_pre(gxv, hp[k].m0, hp[k].q, hp[k].a) : gxv;
                     xu = ihs[k] >= 0 ? pmlT_pre(gxu_, hp[k].m1, hp[k].q, hp[k].a) : gxu_;
@@ -1098,6 +1103,8 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
             }
         }
     }
+    };
+    STFWI_DISPATCH(inx, inz, stage);
     // receiver adjoints of channels vx, vz of this step into the owned
     // U_a, V_a (tiles holding receiver cells only)
     if (tile_receivers(a, s, r0, TZ) > 0) {
@@ -1365,14 +1372,18 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
     }
     // C-PML inputs of the staged cells and the owned cells' G values, kept
     // for the memory update after the barrier (tiles touching a strip only)
-    PmlIn<T> pz[KZ], px[KZ];
+    PmlIn<T> pz[KZ] = {}, px[KZ] = {};
     int izs[KZ], ixs[KZ];
     T gzbv_o[RPT], gzmes_o[RPT], gxbu_o[RPT], gxmes_o[RPT];
 #pragma unroll
     for (int k = 0; k < KZ; ++k) izs[k] = ixs[k] = -1;
 #pragma unroll
     for (int rr = 0; rr < RPT; ++rr) gzbv_o[rr] = gzmes_o[rr] = gxbu_o[rr] = gxmes_o[rr] = T(0);
-    if (inx && inz) {
+    // staging, compiled per tile class: along an interior axis the C-PML
+    // inputs and the edge masks of the staged cells are compiled out
+    auto stage = [&](auto tagx, auto tagz) {
+        constexpr bool IX = decltype(tagx)::value, IZ = decltype(tagz)::value;
+    if (IX && IZ) {
         // no PML, no masks: Dx_(BU)_a = dt/dx B U_a, Dz_M_es_a = dt/dz B U_a, ...
 #pragma unroll
         for (int k = 0; k < KZ; ++k) {
@@ -1402,10 +1413,10 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
 #pragma unroll
         for (int k = 0; k < KZ; ++k) {
             const int r = r0 - 4 + ty + BLOCK_Y * k;
-            izs[k] = zok[k] ? zstrip(g, r, c) : -1;
-            ixs[k] = zok[k] ? xstrip(g, r, c) : -1;
-            pz[k].template load<false>(S.pzo + 2 * S.pzn, S.pzo + 3 * S.pzn, izs[k], a.profz, r, nz);
-            if (k >= 1 && k < TZ / BLOCK_Y + 1)
+            izs[k] = (!IZ && zok[k]) ? zstrip(g, r, c) : -1;
+            ixs[k] = (!IX && zok[k]) ? xstrip(g, r, c) : -1;
+            if (!IZ) pz[k].template load<false>(S.pzo + 2 * S.pzn, S.pzo + 3 * S.pzn, izs[k], a.profz, r, nz);
+            if (!IX && k >= 1 && k < TZ / BLOCK_Y + 1)
                 px[k].template load<false>(S.pxo + 2 * S.pxn, S.pxo + 3 * S.pxn, ixs[k], a.profx, c, nx);
         }
         PmlIn<T> hp[KX];
@@ -1414,15 +1425,15 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
         for (int k = 0; k < KX; ++k) {
             const int e = tid + 256 * k;
             const int zi = e >> 3, h = e & 7, xi = h < 4 ? h : h + TILE_X;
-            ihs[k] = hok[k] ? xstrip(g, r0 + zi, c0 - 4 + xi) : -1;
-            hp[k].template load<false>(S.pxo + 2 * S.pxn, S.pxo + 3 * S.pxn, ihs[k], a.profx, c0 - 4 + xi, nx);
+            ihs[k] = (!IX && hok[k]) ? xstrip(g, r0 + zi, c0 - 4 + xi) : -1;
+            if (!IX) hp[k].template load<false>(S.pxo + 2 * S.pxn, S.pxo + 3 * S.pxn, ihs[k], a.profx, c0 - 4 + xi, nx);
         }
 #pragma unroll
         for (int k = 0; k < KZ; ++k) {
             const int zi = ty + BLOCK_Y * k, r = r0 - 4 + zi;
             T bv = 0, mz = 0, bu = 0, mx = 0;
             if (zok[k]) {
-                const bool inR = r <= nz - 2 && c >= 1 && c <= nx - 2;
+                const bool inR = CZ(r <= nz - 2) && CX(c >= 1 && c <= nx - 2);
                 const T b = zb[k], u = zu[k], v = zv[k];
                 const T gzbv = inR ? g.dtdz * b * v : T(0);
                 const T gzmes = inR ? g.dtdz * b * u : T(0);
@@ -1456,7 +1467,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
                 const int r = r0 + zi, cc = c0 - 4 + xi;
                 T bu = 0, mx = 0;
                 if (hok[k]) {
-                    const bool inR = r <= nz - 2 && cc >= 1 && cc <= nx - 2;
+                    const bool inR = CZ(r <= nz - 2) && CX(cc >= 1 && cc <= nx - 2);
                     const T b = hb[k];
                     const T gxbu = inR ? g.dtdx * b * hu[k] : T(0);
                     const T gxmes = inR ? g.dtdx * b * hv[k] : T(0);
@@ -1468,6 +1479,8 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
             }
         }
     }
+    };
+    STFWI_DISPATCH(inx, inz, stage);
     __syncthreads();
     auto body = [&](auto tagx, auto tagz) {
         constexpr bool IX = decltype(tagx)::value, IZ = decltype(tagz)::value;
