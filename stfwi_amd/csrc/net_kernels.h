@@ -225,7 +225,6 @@ __global__ __launch_bounds__(256) void k_conv_tile(const float *__restrict__ in,
     for (int m = 0; m < 2; ++m)
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt) acc[m][nt] = nf32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
     for (int ks = 0; ks < Kp / 32; ++ks) {
         const int bo = boff[4 * ks + grp];
         nbf16x8 bb[4];
