@@ -141,6 +141,15 @@ __device__ __forceinline__ void stgg(T *base, int idx, T v)
     *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (uint32_t)idx * (uint32_t)sizeof(T)) = v;
 }
 
+// stst: store of a write-once stream (the forward history, snapshots) with
+// the non-temporal hint, so that these streams do not displace the resident
+// wavefield state from the caches; they are read back many steps later.
+template <typename T>
+__device__ __forceinline__ void stst(T *p, T v)
+{
+    __builtin_nontemporal_store(v, p);
+}
+
 // profile layout per axis: kappa[n], b[n], a[n], q[n] with q = 1/kappa + a
 template <typename T>
 __device__ __forceinline__ T pml_fwd(T D, T *psi, const T *prof, int j, int n)
@@ -566,8 +575,8 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
                 const int rp = r - g.z0, cp = c - g.pad;
                 if (CZ(rp >= 0 && rp < g.nzd) && CX(cp >= 0 && cp < g.nxd)) {
                     T *h = a.hist2 + ((i64)s * a.hist_nt + (it - a.hist_t0)) * 2 * g.np + (i64)rp * g.nxd + cp;
-                    h[0] = fma(g.dtdz, D3, fma(g.dtdx, D2, g.dtdx * D1));
-                    h[g.np] = fma(g.dtdz, D6, fma(g.dtdz, D5, g.dtdx * D4)) + sv;
+                    stst<T>(h, fma(g.dtdz, D3, fma(g.dtdx, D2, g.dtdx * D1)));
+                    stst<T>(h + g.np, fma(g.dtdz, D6, fma(g.dtdz, D5, g.dtdx * D4)) + sv);
                 }
             }
         }
@@ -746,13 +755,13 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
             if (a.hist && CZ(rp >= 0 && rp < g.nzd) && CX(cp >= 0 && cp < g.nxd)) {
                 T *hb = a.hist + ((i64)s * a.hist_nt + (it - a.hist_t0)) * 3 * g.np;
                 const int o = rp * g.nxd + cp, npi = (int)g.np;
-                stgg(hb, o, exn);
-                stgg(hb, o + npi, ezn);
-                stgg(hb, o + 2 * npi, esn);
+                stst(hb + o, exn);
+                stst(hb + o + npi, ezn);
+                stst(hb + o + 2 * npi, esn);
                 if (a.hist2_mode == 2) {
                     T *h = a.hist2 + ((i64)s * a.hist_nt + (it - a.hist_t0)) * 2 * g.np + (i64)rp * g.nxd + cp;
-                    h[0] = sU[zc][xc];
-                    h[g.np] = sV[zc][xc];
+                    stst(h, sU[zc][xc]);
+                    stst(h + g.np, sV[zc][xc]);
                 }
             }
             const i64 pc = (i64)r * nx + c;
@@ -765,7 +774,7 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
                 int k = 0;
                 for (int f = 0; f < 5; ++f)
                     if (a.snap_mask & (1 << f)) {
-                        a.snap[(((i64)s * a.nsnap + k) * a.nt_max + it) * nz * nx + pc] = vals[f];
+                        stst(a.snap + (((i64)s * a.nsnap + k) * a.nt_max + it) * nz * nx + pc, vals[f]);
                         ++k;
                     }
             }
@@ -1564,7 +1573,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
                     int k = 0;
                     for (int f = 0; f < 5; ++f)
                         if (a.snap_mask & (1 << f)) {
-                            a.snap[(((i64)s * a.nsnap + k) * a.nt_max + it) * nz * nx + (i64)r * nx + c] = vals[f];
+                            stst(a.snap + (((i64)s * a.nsnap + k) * a.nt_max + it) * nz * nx + (i64)r * nx + c, vals[f]);
                             ++k;
                         }
                 }
