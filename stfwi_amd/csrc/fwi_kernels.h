@@ -1819,6 +1819,52 @@ __global__ __launch_bounds__(64) void k_residual(ResArgs<T> a, double *lpart)
     }
 }
 
+// Without a filter the residual and its transpose are elementwise in time:
+// r = T pred - T obs, then 2 r w / N and T^T, the values k_residual stores.
+// Thread (receiver j, block ch of RES_TCH steps) writes its steps and its
+// block's sum of squares into lpart [shot][chunk][nrec_max].
+#define RES_TCH 64
+template <typename T>
+__global__ __launch_bounds__(64) void k_residual_el(ResArgs<T> a, double *lpart)
+{
+    const int s = blockIdx.z, ch = blockIdx.y, nch = gridDim.y;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= a.nrec_max) return;
+    const ShotDev sh = a.shots[s];
+    const int nt = sh.nt;
+    const i64 chs = (i64)a.nt_max * a.nrec_max;
+    const T *__restrict__ pred = a.pred + ((i64)s * 4 + a.comp) * chs + j;
+    const T *__restrict__ obs = a.obs + (i64)s * chs + j;
+    T *__restrict__ rr = a.rec_adj + ((i64)s * 4 + a.comp) * chs + j;
+    const int t0 = ch * RES_TCH, t1 = t0 + RES_TCH < a.nt_max ? t0 + RES_TCH : a.nt_max;
+    double *lp = lpart + ((i64)s * nch + ch) * a.nrec_max + j;
+    if (j >= sh.nrec) {
+        for (int t = t0; t < t1; ++t) rr[(i64)t * a.nrec_max] = 0;
+        *lp = 0;
+        return;
+    }
+    const T g0 = T(2) * (a.w / (T)((double)nt * sh.nrec));
+    T ssq = 0;
+#pragma unroll 4
+    for (int t = t0; t < t1; ++t) {
+        T out = 0;
+        if (t < nt) {
+            T v0 = pred[(i64)t * a.nrec_max], v1 = obs[(i64)t * a.nrec_max];
+            const T tw = taper_w<T>(t, nt, a.n_taper);
+            if (a.n_taper > 0) {
+                v0 = v0 * tw;
+                v1 = v1 * tw;
+            }
+            const T res = v0 - v1;
+            ssq += res * res;
+            out = res * g0;
+            if (a.n_taper > 0) out = out * tw;
+        }
+        rr[(i64)t * a.nrec_max] = out;
+    }
+    *lp = (double)ssq;
+}
+
 // the other three channels of the receiver adjoints are zero
 template <typename T>
 __global__ void k_residual_zero(T *rec_adj, i64 n, i64 chs, int comp)
@@ -1827,16 +1873,18 @@ __global__ void k_residual_zero(T *rec_adj, i64 n, i64 chs, int comp)
         if ((i / chs) % 4 != comp) rec_adj[i] = 0;
 }
 
-// per-shot misfit: thread t sums receivers t, t + 256, ... in order, then a
+// per-shot misfit from lpart [shot][nch][nrec_max]: thread t sums receivers
+// t, t + 256, ... in order (each over its nch time blocks in order), then a
 // fixed tree (one block per shot)
 template <typename T>
-__global__ __launch_bounds__(256) void k_residual_loss(ResArgs<T> a, const double *lpart)
+__global__ __launch_bounds__(256) void k_residual_loss(ResArgs<T> a, const double *lpart, int nch)
 {
     const int s = blockIdx.x;
     const ShotDev sh = a.shots[s];
     __shared__ double red[256];
     double part = 0;
-    for (int j = threadIdx.x; j < a.nrec_max; j += blockDim.x) part += lpart[(i64)s * a.nrec_max + j];
+    for (int j = threadIdx.x; j < a.nrec_max; j += blockDim.x)
+        for (int c = 0; c < nch; ++c) part += lpart[((i64)s * nch + c) * a.nrec_max + j];
     red[threadIdx.x] = part;
     __syncthreads();
     for (int k = 128; k > 0; k >>= 1) {

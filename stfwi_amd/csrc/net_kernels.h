@@ -262,73 +262,192 @@ __global__ __launch_bounds__(256) void k_conv_tile(const float *__restrict__ in,
     }
 }
 
-// Weight-gradient partials over row chunks: chunk z covers rows
-// [z * rg, (z + 1) * rg) of the batch's stacked rows (row index b * H + y);
-// work[z][co][k] = sum over the chunk's pixels of dout[co][p] * col[k][p],
-// one wave per 16 x 16 (co, k) tile, the pixels of a row taken 32 at a time
-// in order.  Addressing is per row, so no division per element.
-// grid: x = ceil(K / 16), y = ceil(cout / 16), z = ceil(batch * H / rg); 64 threads.
+// Weight gradient: dw[co][k] = sum over pixels p of dout[co][p] col[k][p],
+// k = (ci, ky, kx), as tiled BF16 MFMA GEMMs with FP32 accumulation.  The
+// pixels are cut into tiles of WG_TR rows x 64 columns of one image; block
+// (z, cg, mg) sums tiles z, z + gridDim.x, ... in that order into registers
+// for the input channels [64 cg, 64 cg + 64) and output channels
+// [32 mg, 32 mg + 32), then writes its partial into work[z][co][k] (the
+// partials are summed in order by k_sum_chunks, so the result is
+// deterministic).  Per tile the block stages dout as BF16 [co][pixel] and,
+// per chunk of 16 input channels, the input rows with their halo as BF16
+// [ci][row][position], tile column x at position x + 8.  An operand holds
+// 8 consecutive pixels of one row: the aligned 16-byte read at the tap's
+// unshifted position, and for kx - R = -1 / +1 the neighbouring dword,
+// funnel-shifted by one element.  Wave w owns the N-tiles w, w + 4, ... of
+// the group with both M-tiles, so each B operand feeds two MFMAs.  Every
+// global load reads a clamped in-bounds address and selects zero outside
+// the image, so the staging has no per-lane branches.  Row strides of 264
+// and 88 halfwords are odd multiples of 16 bytes.
+#define WG_TR 4
+#define WG_AS 264
+#define WG_BS 88
+#define WG_NPART 512
 template <int KS>
-__global__ __launch_bounds__(64) void k_conv_wgrad_rows(const float *__restrict__ in, int cin, int H, int W,
-                                                        const float *__restrict__ dout, int cout, int batch, int rg,
-                                                        float *__restrict__ work)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_conv_wgrad_tile(const float *__restrict__ in, int cin, int H, int W,
+                                                         const float *__restrict__ dout, int cout, int batch,
+                                                         float *__restrict__ work)
 {
-    constexpr int K2 = KS * KS, R = KS / 2;
-    const int P = H * W;
-    const int K = cin * K2;
-    const int lane = threadIdx.x;
-    const int k0 = blockIdx.x * 16, co0 = blockIdx.y * 16;
-    const int kr = 8 * (lane >> 4);
-    const int am = co0 + (lane & 15);          // A row: output channel
-    const int bk = k0 + (lane & 15);           // B column: (ci, ky, kx)
-    int bci = 0, bky = 0, bkx = 0;
-    if (bk < K) {
-        bci = bk / K2;
-        const int r = bk - bci * K2;
-        bky = r / KS;
-        bkx = r - bky * KS;
-    }
-    const bool aok = am < cout, bok = bk < K;
-    nf32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    const int row0 = blockIdx.z * rg, row1 = min(row0 + rg, batch * H);
-    for (int row = row0; row < row1; ++row) {
-        const int b = row / H, y = row - b * H;
-        const float *arow = dout + ((long long)b * cout + (aok ? am : 0)) * P + (long long)y * W;
-        const int yy = y + bky - R;
-        const bool rok = bok && yy >= 0 && yy < H;
-        const float *brow = in + ((long long)b * cin + bci) * P + (long long)(rok ? yy : 0) * W;
-        for (int x0 = 0; x0 < W; x0 += 32) {
-            float va[8], vb[8];
+    constexpr int K2 = KS * KS, R = KS / 2, BR = WG_TR + 2 * R;
+    __shared__ __attribute__((aligned(16))) __bf16 As[32 * WG_AS];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[16 * BR * WG_BS];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int P = H * W, K = cin * K2;
+    const int cg = blockIdx.y, mg = blockIdx.z;
+    const int co0 = 32 * mg, nco = cout - co0 < 32 ? cout - co0 : 32;
+    const int nmt = nco > 16 ? 2 : 1;
+    const int ci0 = 64 * cg, nci = cin - ci0 < 64 ? cin - ci0 : 64;
+    const int nchunk = (nci + 15) / 16;
+    const int tiles_y = (H + WG_TR - 1) / WG_TR, tiles_x = (W + 63) / 64;
+    const int ntiles = batch * tiles_y * tiles_x;
+    // the lane's B column in each of the wave's N-tiles: LDS offset of tile
+    // column 0 in staged row ky (-1 past the last column) and the tap's
+    // column shift kx - R
+    int boff[K2], bsh[K2];
+    bool tvalid[K2];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int x = x0 + kr + i, xx = x + bkx - R;
-                va[i] = (aok && x < W) ? arow[x] : 0.f;
-                vb[i] = (rok && x < W && xx >= 0 && xx < W) ? brow[xx] : 0.f;
-            }
-            nbf16x8 a, bb;
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                a[i] = (__bf16)va[i];
-                bb[i] = (__bf16)vb[i];
-            }
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bb, acc, 0, 0, 0);
+    for (int u = 0; u < K2; ++u) {
+        const int tl = wave + 4 * u;
+        const int n = 16 * (cg * 4 * K2 + tl) + (lane & 15);
+        tvalid[u] = tl / K2 < nchunk;
+        boff[u] = -1;
+        bsh[u] = 0;
+        if (tvalid[u] && n < K) {
+            const int ci = n / K2, rem = n - ci * K2;
+            const int ky = rem / KS, kx = rem - ky * KS;
+            boff[u] = ((ci & 15) * BR + ky) * WG_BS + 8;
+            bsh[u] = kx - R;
         }
     }
-    float *wz = work + (long long)blockIdx.z * cout * K;
+    nf32x4 acc[2][K2];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int co = co0 + 4 * (lane >> 4) + i, k = k0 + (lane & 15);
-        if (co < cout && k < K) wz[(long long)co * K + k] = acc[i];
+    for (int u = 0; u < K2; ++u) acc[0][u] = acc[1][u] = nf32x4{0.f, 0.f, 0.f, 0.f};
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int b = tile / (tiles_y * tiles_x), r2 = tile - b * tiles_y * tiles_x;
+        const int y0 = (r2 / tiles_x) * WG_TR, x0 = (r2 - (r2 / tiles_x) * tiles_x) * 64;
+        __syncthreads();   // the previous tile's operand reads are done
+        {
+            // dout rows: thread (row j = wave, column lane), 16 channels at
+            // a time, every load of a group issued before its stores
+            const int y = y0 + wave, x = x0 + lane;
+            const bool ok = y < H && x < W;
+            const float *db = dout + ((long long)b * cout + co0) * P;
+            const int po = ok ? y * W + x : 0;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                float va[16];
+#pragma unroll
+                for (int c = 0; c < 16; ++c) {
+                    const int co = 16 * h + c;
+                    const float t = db[(co < nco ? co : 0) * P + po];
+                    va[c] = (ok && co < nco) ? t : 0.f;
+                }
+#pragma unroll
+                for (int c = 0; c < 16; ++c) As[(16 * h + c) * WG_AS + wave * 64 + lane] = (__bf16)va[c];
+            }
+        }
+        for (int cc = 0; cc < nchunk; ++cc) {
+            if (cc > 0) __syncthreads();   // the previous chunk's B reads are done
+            // rows y0 - R + row of channel ci0 + 16 cc + cil (pr = cil BR +
+            // row): thread (wave, lane) takes the rows pr = wave, wave + 4,
+            // ... at tile column lane - R, half of them at a time; thread t
+            // then takes extra column e = t mod 2R (tile column 64 + e - R)
+            // of row t / 2R
+            constexpr int NPR = 4 * BR;
+            const float *cb = in + ((long long)b * cin + ci0 + 16 * cc) * P;
+            const int nrow = nci - 16 * cc;   // channels present in this chunk
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                float vb[NPR / 2];
+#pragma unroll
+                for (int m = 0; m < NPR / 2; ++m) {
+                    const int pr = wave + 4 * (h * (NPR / 2) + m), cil = pr / BR, row = pr - cil * BR;
+                    const int y = y0 + row - R, xa = x0 - R + lane;
+                    const bool oka = cil < nrow && y >= 0 && y < H && xa >= 0 && xa < W;
+                    const float t = cb[oka ? cil * P + y * W + xa : 0];
+                    vb[m] = oka ? t : 0.f;
+                }
+#pragma unroll
+                for (int m = 0; m < NPR / 2; ++m) {
+                    const int pr = wave + 4 * (h * (NPR / 2) + m);
+                    Bs[pr * WG_BS + lane - R + 8] = (__bf16)vb[m];
+                }
+            }
+            if (R > 0) {
+                const bool act = tid < 16 * BR * 2 * R;
+                const int pr = act ? tid / (2 * R) : 0, e = tid - (tid / (2 * R)) * (2 * R);
+                const int cil = pr / BR, row = pr - cil * BR;
+                const int y = y0 + row - R, xe = x0 - R + 64 + e;
+                const bool oke = act && cil < nrow && y >= 0 && y < H && xe < W;
+                const float t = cb[oke ? cil * P + y * W + xe : 0];
+                Bs[pr * WG_BS + (act ? 64 + e - R + 8 : 80 + (lane & 7))] = (__bf16)(oke ? t : 0.f);
+            }
+            __syncthreads();
+#pragma unroll 1
+            for (int ks = 0; ks < 2 * WG_TR; ++ks) {
+                const int j = ks >> 1, xb = (ks & 1) * 32 + 8 * (lane >> 4);
+                const nbf16x8 a0 = *reinterpret_cast<const nbf16x8 *>(As + (lane & 15) * WG_AS + j * 64 + xb);
+                const nbf16x8 a1 = *reinterpret_cast<const nbf16x8 *>(As + (16 + (lane & 15)) * WG_AS + j * 64 + xb);
+#pragma unroll
+                for (int u = 0; u < K2; ++u) {
+                    if (!tvalid[u] || (wave + 4 * u) / K2 != cc) continue;
+                    const int base = (boff[u] >= 0 ? boff[u] : 8) + j * WG_BS + xb;
+                    uint4 q = *reinterpret_cast<const uint4 *>(Bs + base);
+                    if (R > 0) {
+                        const unsigned pv = *reinterpret_cast<const unsigned *>(Bs + base - 2);
+                        const unsigned nv = *reinterpret_cast<const unsigned *>(Bs + base + 8);
+                        const unsigned s0 = __builtin_amdgcn_alignbyte(q.x, pv, 2);
+                        const unsigned s1 = __builtin_amdgcn_alignbyte(q.y, q.x, 2);
+                        const unsigned s2 = __builtin_amdgcn_alignbyte(q.z, q.y, 2);
+                        const unsigned s3 = __builtin_amdgcn_alignbyte(q.w, q.z, 2);
+                        const unsigned s4 = __builtin_amdgcn_alignbyte(nv, q.w, 2);
+                        if (bsh[u] < 0) q = uint4{s0, s1, s2, s3};
+                        else if (bsh[u] > 0) q = uint4{s1, s2, s3, s4};
+                    }
+                    if (boff[u] < 0) q = uint4{0u, 0u, 0u, 0u};
+                    const nbf16x8 bb = __builtin_bit_cast(nbf16x8, q);
+                    acc[0][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bb, acc[0][u], 0, 0, 0);
+                    if (nmt > 1) acc[1][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bb, acc[1][u], 0, 0, 0);
+                }
+            }
+        }
+    }
+    float *wz = work + (long long)blockIdx.x * cout * K;
+#pragma unroll
+    for (int u = 0; u < K2; ++u) {
+        const int k = 16 * (cg * 4 * K2 + wave + 4 * u) + (lane & 15);
+        if (!tvalid[u] || k >= K) continue;
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int co = 16 * mt + 4 * (lane >> 4) + i;
+                if (mt < nmt && co < nco) wz[(long long)(co0 + co) * K + k] = acc[mt][u][i];
+            }
     }
 }
 
-// out[i] = sum_z work[z][i] (fixed order), optionally added to out
-__global__ void k_sum_chunks(const float *__restrict__ work, int nz, long long n, float *__restrict__ out, int add)
+// out[i] = sum_z work[z][i], optionally added to out.  256 threads per 64
+// outputs: quarter q of the threads sums z in [q nz / 4, (q + 1) nz / 4) in
+// order, then the four quarter sums are added in order (a fixed order, so
+// the result is deterministic).  grid: ceil(n / 64) blocks of 256 threads.
+__global__ __launch_bounds__(256) void k_sum_chunks(const float *__restrict__ work, int nz, long long n,
+                                                    float *__restrict__ out, int add)
 {
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-        float s = 0.f;
-        for (int z = 0; z < nz; ++z) s += work[(long long)z * n + i];
-        out[i] = add ? out[i] + s : s;
+    __shared__ float red[4][64];
+    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const long long i = (long long)blockIdx.x * 64 + lane;
+    const int z0 = (int)((long long)nz * q / 4), z1 = (int)((long long)nz * (q + 1) / 4);
+    float sum = 0.f;
+    if (i < n) {
+#pragma unroll 8
+        for (int z = z0; z < z1; ++z) sum += work[(long long)z * n + i];
+    }
+    red[q][lane] = sum;
+    __syncthreads();
+    if (q == 0 && i < n) {
+        const float t = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+        out[i] = add ? out[i] + t : t;
     }
 }
 

@@ -90,7 +90,7 @@ struct stfwi_plan {
     // every cell to its merged receiver index (-1 if none), per-row prefix
     // counts of merged receiver cells, and the per-step merged sums
     int *cellmap_d = nullptr, *rowcum_d = nullptr;
-    double *lpart = nullptr;   // [shot][nrec_max] receiver sums of squares of the residual
+    double *lpart = nullptr;   // [shot][time blocks][nrec_max] sums of squares of the residual
     // adjoint energy accumulator of the next stfwi_adjoint (caller-owned)
     void *aen = nullptr;
     int aen_comp = 0;
@@ -256,7 +256,8 @@ extern "C" int stfwi_plan_create(const stfwi_config *cfg, stfwi_plan **out)
     rc |= dalloc(p, (void **)&p->uc_rec_d, (S * std::max(c.nrec_max, 1)) * sizeof(int));
     rc |= dalloc(p, (void **)&p->cellmap_d, S * p->fstride * sizeof(int));
     rc |= dalloc(p, (void **)&p->rowcum_d, S * (p->nz + 1) * sizeof(int));
-    rc |= dalloc(p, (void **)&p->lpart, S * std::max(c.nrec_max, 1) * sizeof(double));
+    rc |= dalloc(p, (void **)&p->lpart,
+                 S * std::max(c.nrec_max, 1) * ((std::max(c.nt_max, 1) + RES_TCH - 1) / RES_TCH) * sizeof(double));
     rc |= dalloc(p, &p->injm, (size_t)S * c.nt_max * 4 * std::max(c.nrec_max, 1) * es);
     if (rc) {
         std::string e = g_err;
@@ -870,8 +871,16 @@ static int residual_t(stfwi_plan *p, const void *pred, const void *obs, int comp
     const i64 chs = (i64)a.nt_max * a.nrec_max, n = (i64)p->n_shots * 4 * chs;
     hipLaunchKernelGGL(k_residual_zero<T>, dim3((int)std::min<i64>((n + 255) / 256, 16384)), dim3(256), 0, st,
                        (T *)rec_adj, n, chs, comp);
-    hipLaunchKernelGGL(k_residual<T>, dim3((a.nrec_max + 63) / 64, p->n_shots), dim3(64), 0, st, a, p->lpart);
-    hipLaunchKernelGGL(k_residual_loss<T>, dim3(p->n_shots), dim3(256), 0, st, a, (const double *)p->lpart);
+    int nch = 1;
+    if (n_sos == 0) {
+        // no filter: parallel over blocks of time steps as well
+        nch = (a.nt_max + RES_TCH - 1) / RES_TCH;
+        hipLaunchKernelGGL(k_residual_el<T>, dim3((a.nrec_max + 63) / 64, nch, p->n_shots), dim3(64), 0, st, a,
+                           p->lpart);
+    } else {
+        hipLaunchKernelGGL(k_residual<T>, dim3((a.nrec_max + 63) / 64, p->n_shots), dim3(64), 0, st, a, p->lpart);
+    }
+    hipLaunchKernelGGL(k_residual_loss<T>, dim3(p->n_shots), dim3(256), 0, st, a, (const double *)p->lpart, nch);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -1383,7 +1392,7 @@ extern "C" int stfwi_conv2d_bf16_dgrad(const float *dout, int batch, int cout, i
 
 extern "C" long long stfwi_conv2d_bf16_wgrad_work(int cin, int cout, int ksize)
 {
-    return 256LL * cout * cin * ksize * ksize;
+    return (long long)WG_NPART * cout * cin * ksize * ksize;
 }
 
 extern "C" int stfwi_conv2d_bf16_wgrad(const float *in, const float *dout, int batch, int cin, int cout, int h, int w,
@@ -1397,17 +1406,17 @@ extern "C" int stfwi_conv2d_bf16_wgrad(const float *in, const float *dout, int b
     if (work_elems < stfwi_conv2d_bf16_wgrad_work(cin, cout, ksize))
         return fail(STFWI_ERR_INVALID, "work buffer holds %lld elements, needs %lld", work_elems,
                     stfwi_conv2d_bf16_wgrad_work(cin, cout, ksize));
-    // up to 256 chunks of whole image rows
+    // up to WG_NPART partials, each over a strided set of 4 x 64 pixel tiles
     hipStream_t st = (hipStream_t)stream;
-    const int rows = batch * h, rg = (rows + 255) / 256;
-    const int nz = (rows + rg - 1) / rg;
-    dim3 grid((unsigned)((K + 15) / 16), (unsigned)((cout + 15) / 16), (unsigned)nz);
+    const long long ntiles = (long long)batch * ((h + WG_TR - 1) / WG_TR) * ((w + 63) / 64);
+    const int nz = (int)std::min<long long>(ntiles, WG_NPART);
+    dim3 grid((unsigned)nz, (unsigned)((cin + 63) / 64), (unsigned)((cout + 31) / 32));
     if (ksize == 3)
-        hipLaunchKernelGGL(k_conv_wgrad_rows<3>, grid, dim3(64), 0, st, in, cin, h, w, dout, cout, batch, rg, work);
+        hipLaunchKernelGGL(k_conv_wgrad_tile<3>, grid, dim3(256), 0, st, in, cin, h, w, dout, cout, batch, work);
     else
-        hipLaunchKernelGGL(k_conv_wgrad_rows<1>, grid, dim3(64), 0, st, in, cin, h, w, dout, cout, batch, rg, work);
+        hipLaunchKernelGGL(k_conv_wgrad_tile<1>, grid, dim3(256), 0, st, in, cin, h, w, dout, cout, batch, work);
     const long long n = (long long)cout * K;
-    hipLaunchKernelGGL(k_sum_chunks, dim3(grid1(n)), dim3(256), 0, st, work, nz, n, dw, accumulate);
+    hipLaunchKernelGGL(k_sum_chunks, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, st, work, nz, n, dw, accumulate);
     if (db) {
         // bias gradient: per-chunk channel sums into work (after the
         // weight partials are summed), then summed in chunk order into db
@@ -1415,7 +1424,8 @@ extern "C" int stfwi_conv2d_bf16_wgrad(const float *in, const float *dout, int b
         const int nb = (int)std::min<long long>((PT + bch - 1) / bch, 256);
         const long long pchb = (PT + nb - 1) / nb;
         hipLaunchKernelGGL(k_bias_grad, dim3(cout, nb), dim3(256), 0, st, dout, cout, batch, h * w, (int)pchb, work);
-        hipLaunchKernelGGL(k_sum_chunks, dim3(grid1(cout)), dim3(256), 0, st, work, nb, (long long)cout, db, accumulate);
+        hipLaunchKernelGGL(k_sum_chunks, dim3((unsigned)((cout + 63) / 64)), dim3(256), 0, st, work, nb, (long long)cout, db,
+                           accumulate);
     }
     HIP_TRY(hipGetLastError());
     return 0;
@@ -1474,7 +1484,7 @@ extern "C" int stfwi_mse(const float *y, const float *t, const float *wgt, long 
     if (!y || !t || !g || !part || !loss || n < 1) return fail(STFWI_ERR_INVALID, "bad argument");
     hipStream_t st = (hipStream_t)stream;
     hipLaunchKernelGGL(k_mse, dim3(256), dim3(256), 0, st, y, t, wgt, n, (float)inv_wsum, g, part);
-    hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(1), 0, st, part, 256, 1LL, loss, 0);
+    hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(256), 0, st, part, 256, 1LL, loss, 0);
     HIP_TRY(hipGetLastError());
     return 0;
 }

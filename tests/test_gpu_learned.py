@@ -20,7 +20,7 @@ def _rel(a, b):
 
 
 @pytest.mark.parametrize("k,B,cin,cout,h,w", [(3, 2, 5, 19, 13, 21), (3, 1, 16, 32, 24, 40), (1, 2, 16, 1, 10, 14),
-                                              (3, 1, 48, 16, 8, 6)])
+                                              (3, 1, 48, 16, 8, 6), (3, 2, 70, 40, 9, 70), (1, 1, 80, 33, 5, 130)])
 def test_conv_forward_and_gradients_match_cpu(k, B, cin, cout, h, w):
     """Each kernel against the same convolution evaluated in float64 on the
     BF16-rounded operands it consumes: only the FP32 accumulation differs."""

@@ -322,3 +322,30 @@ def test_gpu_adjoint_energy_equals_sum_of_squared_snapshots(comp):
     assert np.all(e >= 0) and np.any(e > 0)
     again = p.adjoint(ra)
     assert "energy" not in again
+
+
+@pytest.mark.parametrize("ntap", [0, 12])
+def test_gpu_residual_unfiltered_ragged_f64(ntap):
+    """Without a filter the residual runs over blocks of 64 time steps: shots
+    whose record lengths are no multiple of 64 and whose receiver counts are
+    below the plan's maximum match the oracle's residual shot by shot, and the
+    steps past a shot's record and its absent receivers are zero."""
+    m = dp_model()
+    nt_max, nts, nrecs = 150, [150, 97], [7, 4]
+    rng = np.random.default_rng(11)
+    rxl = [np.arange(2, 2 + 3 * n, 3) for n in nrecs]
+    rzl = [np.full(n, 3) for n in nrecs]
+    p = _plan(m, 4, True, torch.float64, nshots=2, nrec=max(nrecs), nt=nt_max)
+    p.set_model(m["Vs"], m["Vp"], m["rho"])
+    p.set_shots([10, 20], [5, 6], "force", nts, rxl, rzl, np.zeros(nt_max))
+    pred = rng.standard_normal((2, 4, nt_max, max(nrecs)))
+    obs = rng.standard_normal((2, nt_max, max(nrecs)))
+    loss, radj = p.residual(torch.as_tensor(pred), torch.as_tensor(obs), 1, None, ntap, 3.0, 2)
+    radj = radj.cpu().numpy()
+    for k in range(2):
+        nt, nr = nts[k], nrecs[k]
+        l_o, cot = osig.residual_prep(pred[k, 1, :nt, :nr], obs[k, :nt, :nr], None, ntap, 3.0, 2)
+        assert abs(float(loss[k]) / l_o - 1) < TOL_F64, k
+        assert rel_l2(radj[k, 1, :nt, :nr], cot) < TOL_F64, k
+        assert not radj[k, 1, nt:].any() and not radj[k, 1, :, nr:].any(), k
+        assert not radj[k, [0, 2, 3]].any(), k
