@@ -122,3 +122,35 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_marmousi_survey_matches_reference_fixtures(tmp_path):
+    """The drivers' Marmousi set-up against what the reference stored with
+    its gradients: the 34- and 5-shot source lines, the 250 surface
+    receivers and the time step (to the last bit) and step count of the 6 s
+    record; the loader reads the reference's .npy layout, and smooth_initial
+    with sigma 20 gives the SHA-pinned initial models."""
+    from stfwi_amd.models import load_marmousi, marmousi_acquisition, smooth_initial, stable_dt
+    from tests.common import load_golden, marmousi_init
+    t = load_golden("marmousi_true.npz")
+    for name, n in (("first_gradient.npz", 34), ("gradient_compare.npz", 5)):
+        g = load_golden(name)
+        sx, sz, rx, rz = marmousi_acquisition(int(g["nx"]), n)
+        assert np.array_equal(sx, g["src_x"]) and np.array_equal(sz, g["src_z"]), name
+        assert np.array_equal(rx, g["rec_x"]) and np.array_equal(rz, g["rec_z"]), name
+        dt = stable_dt(t["vp"].max(), float(g["dx"]), float(g["dz"]), int(g["fd_order"]))
+        assert dt == float(g["dt"]), name
+    fg = load_golden("first_gradient.npz")
+    assert int(6.0 / float(fg["dt"])) == int(fg["nt"])
+    for k, a in (("vs_true", t["vs"]), ("vp_true", t["vp"]), ("rho_true", t["rho"])):
+        np.save(tmp_path / (k + ".npy"), a)
+    m = load_marmousi(str(tmp_path))
+    assert sorted(m) == ["rho_true", "vp_true", "vs_true"]
+    assert np.array_equal(m["vp_true"], t["vp"]) and m["vp_true"].dtype == t["vp"].dtype
+    ref = marmousi_init()
+    for a, k in zip(smooth_initial(m["vs_true"], m["vp_true"], m["rho_true"], sigma=20),
+                    ("vs_init", "vp_init", "rho_init")):
+        assert np.array_equal(a, ref[k]), k
+    os.remove(tmp_path / "rho_true.npy")
+    with pytest.raises(FileNotFoundError):
+        load_marmousi(str(tmp_path))
