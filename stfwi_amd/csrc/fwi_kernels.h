@@ -297,6 +297,34 @@ __global__ void k_fill(T *p, i64 n, T v)
 // ---------------------------------------------------------------------------
 // forward
 // ---------------------------------------------------------------------------
+// Division by a launch-constant divisor without a divide sequence
+// (Granlund and Montgomery, "Division by invariant integers using
+// multiplication", PLDI 1994, Fig. 4.1): for 1 <= d < 2^31 and
+// l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1; then for every 32-bit
+// n, n / d = (t + ((n - t) >> s1)) >> s2 with t = mulhi(m, n), s1 = min(l, 1)
+// and s2 = max(l - 1, 0).  The host builds it once per launch; in a kernel
+// it is five scalar instructions.
+struct UDiv {
+    unsigned m, s1, s2;
+};
+
+inline UDiv make_udiv(unsigned d)
+{
+    unsigned l = 0;
+    while ((1ull << l) < d) ++l;
+    UDiv u;
+    u.m = (unsigned)((((1ull << 32) * ((1ull << l) - d)) / d) + 1);
+    u.s1 = l < 1 ? l : 1;
+    u.s2 = l > 1 ? l - 1 : 0;
+    return u;
+}
+
+__device__ __forceinline__ unsigned udiv(unsigned n, const UDiv &u)
+{
+    const unsigned t = __umulhi(u.m, n);
+    return (t + ((n - t) >> u.s1)) >> u.s2;
+}
+
 template <typename T>
 struct FwdArgs {
     Geo<T> g;
@@ -323,6 +351,7 @@ struct FwdArgs {
     int snap_mask, nsnap;
     int tiles_x, tiles_z, ntiles;
     int shot0;                         // first shot of this launch's group
+    UDiv dgx, dtx;                     // division by gridDim.x and by tiles_x
 };
 
 #define TILE_X 64
@@ -345,7 +374,8 @@ struct FwdArgs {
 // each XCD a contiguous range of logical blocks; neighbouring tiles (which
 // share halos) and whole shots then stay in one XCD's L2.  The map is a
 // bijection on [0, gridDim.x * gridDim.z) whatever the dispatch order.
-__device__ __forceinline__ void logical_block(int &bx, int &bz)
+// dgx divides by gridDim.x.
+__device__ __forceinline__ void logical_block(const UDiv &dgx, int &bx, int &bz)
 {
     const int gx = gridDim.x;
     const int n = gx * gridDim.z;
@@ -353,7 +383,7 @@ __device__ __forceinline__ void logical_block(int &bx, int &bz)
     const int q = n >> 3, rmd = n & 7;
     const int xcd = L & 7, w = L >> 3;
     const int lg = xcd < rmd ? xcd * (q + 1) + w : rmd * (q + 1) + (xcd - rmd) * q + w;
-    bz = lg / gx;
+    bz = (int)udiv((unsigned)lg, dgx);
     bx = lg - bz * gx;
 }
 
@@ -390,7 +420,7 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
 {
     const Geo<T> &g = a.g;
     int bx, s;
-    logical_block(bx, s);
+    logical_block(a.dgx, bx, s);
     s += a.shot0;
     const ShotDev sh = a.shots[s];
     if (bx >= a.ntiles) {
@@ -410,7 +440,7 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
     __shared__ T sP3z[TZ + 8][TILE_X], sP4[TZ + 8][TILE_X], sP5[TZ + 8][TILE_X];
     constexpr int NK = FDC<FD>::NK;
     const int ld = g.ld, nz = g.nz, nx = g.nx;
-    const int tz = bx / a.tiles_x, tx = bx - tz * a.tiles_x;
+    const int tz = (int)udiv((unsigned)bx, a.dtx), tx = bx - tz * a.tiles_x;
     const int r0 = tz * TZ, c0 = tx * TILE_X;
     const int lx = threadIdx.x, ty = threadIdx.y, tid = ty * TILE_X + lx;
     T *st = a.state + s * a.sstride + g.origin;
@@ -591,7 +621,7 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
 {
     const Geo<T> &g = a.g;
     int bx, s;
-    logical_block(bx, s);
+    logical_block(a.dgx, bx, s);
     s += a.shot0;
     const ShotDev sh = a.shots[s];
     if (it >= sh.nt) return;
@@ -609,7 +639,7 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
     __shared__ T sU[TZ + 9][TILE_X + 8], sV[TZ + 9][TILE_X + 8];
     constexpr int NK = FDC<FD>::NK;
     const int ld = g.ld, nz = g.nz, nx = g.nx;
-    const int tz = bx / a.tiles_x, tx = bx - tz * a.tiles_x;
+    const int tz = (int)udiv((unsigned)bx, a.dtx), tx = bx - tz * a.tiles_x;
     const int r0 = tz * TZ, c0 = tx * TILE_X;
     const int lx = threadIdx.x, ty = threadIdx.y, tid = ty * TILE_X + lx;
     T *st = a.state + s * a.sstride + g.origin;
@@ -821,6 +851,7 @@ struct AdjArgs {
     int aen_comp;
     int tiles_x, tiles_z, ntiles;
     int shot0;                         // first shot of this launch's group
+    UDiv dgx, dtx;                     // division by gridDim.x and by tiles_x
 };
 
 // Receiver-adjoint injection (fwi/adjoint_jax_kernel.py:201-206; duplicate
@@ -943,7 +974,7 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
 {
     const Geo<T> &g = a.g;
     int bx, s;
-    logical_block(bx, s);
+    logical_block(a.dgx, bx, s);
     s += a.shot0;
     const ShotDev sh = a.shots[s];
     if (it >= sh.nt) return;
@@ -952,7 +983,7 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
     __shared__ T sxV[TZ + 1][TILE_X + 8], sxU[TZ + 1][TILE_X + 8];
     constexpr int NK = FDC<FD>::NK;
     const int nz = g.nz, nx = g.nx, ld = g.ld;
-    const int tz = bx / a.tiles_x, tx = bx - tz * a.tiles_x;
+    const int tz = (int)udiv((unsigned)bx, a.dtx), tx = bx - tz * a.tiles_x;
     const int r0 = tz * TZ, c0 = tx * TILE_X;
     const int lx = threadIdx.x, ty = threadIdx.y, tid = ty * TILE_X + lx;
     T *st = a.state + s * a.sstride + g.origin;
@@ -1273,7 +1304,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
 {
     const Geo<T> &g = a.g;
     int bx, s;
-    logical_block(bx, s);
+    logical_block(a.dgx, bx, s);
     s += a.shot0;
     const ShotDev sh = a.shots[s];
     if (it >= sh.nt) return;
@@ -1282,7 +1313,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
     __shared__ T szBV[TZ + 8][TILE_X], szMes[TZ + 8][TILE_X];
     constexpr int NK = FDC<FD>::NK;
     const int nz = g.nz, nx = g.nx, ld = g.ld;
-    const int tz = bx / a.tiles_x, tx = bx - tz * a.tiles_x;
+    const int tz = (int)udiv((unsigned)bx, a.dtx), tx = bx - tz * a.tiles_x;
     const int r0 = tz * TZ, c0 = tx * TILE_X;
     const int lx = threadIdx.x, ty = threadIdx.y, tid = ty * TILE_X + lx;
     T *st = a.state + s * a.sstride + g.origin;

@@ -414,6 +414,8 @@ static void launch_fwd(int which, int tz, FwdArgs<T> a, int rblk, int S, hipStre
 {
     a.tiles_z = (a.g.nz + tz - 1) / tz;
     a.ntiles = a.tiles_x * a.tiles_z;
+    a.dgx = make_udiv((unsigned)(a.ntiles + rblk));
+    a.dtx = make_udiv((unsigned)a.tiles_x);
     dim3 grid(a.ntiles + rblk, 1, S), blk(TILE_X, BLOCK_Y);
 #define STFWI_LF(TZV)                                                                         \
     do {                                                                                      \
@@ -437,6 +439,8 @@ static void launch_adj(int which, int tz, AdjArgs<T> a, int S, hipStream_t st, i
 {
     a.tiles_z = (a.g.nz + tz - 1) / tz;
     a.ntiles = a.tiles_x * a.tiles_z;
+    a.dgx = make_udiv((unsigned)a.ntiles);
+    a.dtx = make_udiv((unsigned)a.tiles_x);
     dim3 grid(a.ntiles, 1, S), blk(TILE_X, BLOCK_Y);
 #define STFWI_LA(TZV)                                                                          \
     do {                                                                                       \
