@@ -128,17 +128,28 @@ __device__ __forceinline__ void stg(T *base, int idx, T v)
 // 32-bit unsigned byte offset from the uniform base.  The velocity-forward
 // and strain-adjoint kernels use the buffer form, the strain-forward and
 // velocity-adjoint kernels this one (the faster of the two for each, as
-// timed on an MI355X).
+// timed on an MI355X).  The offset passes through an empty asm statement
+// (no instruction; it only hides how the offset was computed), so that the
+// compiler keeps it as a 32-bit VGPR and selects the SGPR-base addressing
+// form instead of a 64-bit vector add per access.
+__device__ __forceinline__ uint32_t opaque_off(uint32_t q)
+{
+    asm("" : "+v"(q));
+    return q;
+}
+
 template <typename T>
 __device__ __forceinline__ T ldgg(const T *base, int idx)
 {
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + (uint32_t)idx * (uint32_t)sizeof(T));
+    const uint32_t q = opaque_off((uint32_t)idx * (uint32_t)sizeof(T));
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + q);
 }
 
 template <typename T>
 __device__ __forceinline__ void stgg(T *base, int idx, T v)
 {
-    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (uint32_t)idx * (uint32_t)sizeof(T)) = v;
+    const uint32_t q = opaque_off((uint32_t)idx * (uint32_t)sizeof(T));
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + q) = v;
 }
 
 // stst: store of a write-once stream (the forward history, snapshots) with
