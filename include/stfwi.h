@@ -246,6 +246,15 @@ int stfwi_set_adjoint_energy(stfwi_plan *plan, void *energy, int comp);
  * back into the caller's stream; results do not depend on the grouping. */
 int stfwi_set_groups(stfwi_plan *plan, int groups);
 
+/* No reference counterpart (a diagnostic).  Each time-step kernel has a
+ * specialised instantiation for runs without its optional outputs (velocity
+ * history, illumination energy, snapshots, adjoint energy, source-wavelet
+ * adjoint) and, for the velocity adjoint, with exact imaging of Vs only;
+ * the launches pick it when the run's arguments match.  enable = 0 makes
+ * every launch use the general instantiation, whose results are the same
+ * bit for bit.  Default 1. */
+int stfwi_set_lean(stfwi_plan *plan, int enable);
+
 /* ---- L-BFGS in float64 on the device (Nocedal & Wright Alg. 7.4).  The
  * reference runs its quasi-Newton step on the host: scipy L-BFGS-B in
  * run_multiscale_fwi (fwi/optim_lbfgsb.py:409-434) and jaxopt.LBFGS in

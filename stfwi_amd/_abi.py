@@ -35,6 +35,7 @@ SIGNATURES = {
     "stfwi_autotune": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP]),
     "stfwi_set_tiles": (C.c_int, [_VP, _VP]),
     "stfwi_set_groups": (C.c_int, [_VP, C.c_int]),
+    "stfwi_set_lean": (C.c_int, [_VP, C.c_int]),
     "stfwi_set_graphs": (C.c_int, [_VP, C.c_int]),
     "stfwi_smooth2d": (C.c_int, [_VP, _VP, _VP, C.c_int, C.c_int, C.c_double, C.c_int, _VP]),
     "stfwi_illumination": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,

@@ -363,11 +363,18 @@ struct FwdArgs {
     int tiles_x, tiles_z, ntiles;
     int shot0;                         // first shot of this launch's group
     UDiv dgx, dtx;                     // division by gridDim.x and by tiles_x
+    int lean;                          // the LEAN instantiations may be used
 };
 
 #define TILE_X 64
 #define BLOCK_Y 4
 
+// Each time-step kernel also has a LEAN instantiation for the common case
+// of a gradient (or a forward) without the optional outputs: the launch
+// picks it when the arguments already hold the values it fixes, so both
+// instantiations compute the same thing; fixing them at compile time drops
+// the unused paths and frees the scalar registers they held.
+//
 // Every time-step kernel works on a TZ x TILE_X tile (TZ = 8, 16 or 32, a
 // template parameter the autotuner picks) with 256 threads (64 x 4, each
 // thread owns TZ/4 rows).  Phase 1 stages what the stencils read (stress
@@ -426,9 +433,13 @@ __device__ __forceinline__ bool interior_z(const Geo<T> &g, int r0, int tz)
 
 // velocity half-step (fwi/forward.py:253-367) + exx/ezz receivers of the
 // previous step (strains are read-only here)
-template <typename T, int FD, int TZ>
+template <typename T, int FD, int TZ, bool LEAN>
 __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
 {
+    if constexpr (LEAN) {
+        // no velocity history
+        a.hist2_mode = 0;
+    }
     const Geo<T> &g = a.g;
     int bx, s;
     logical_block(a.dgx, bx, s);
@@ -627,9 +638,15 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
 
 // strain half-step (fwi/forward.py:369-444) + vx/vz receivers, history,
 // illumination energy, snapshots
-template <typename T, int FD, int TZ>
+template <typename T, int FD, int TZ, bool LEAN>
 __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
 {
+    if constexpr (LEAN) {
+        // no illumination energy, snapshots or velocity history
+        a.illum = nullptr;
+        a.snap = nullptr;
+        a.hist2_mode = 0;
+    }
     const Geo<T> &g = a.g;
     int bx, s;
     logical_block(a.dgx, bx, s);
@@ -863,6 +880,7 @@ struct AdjArgs {
     int tiles_x, tiles_z, ntiles;
     int shot0;                         // first shot of this launch's group
     UDiv dgx, dtx;                     // division by gridDim.x and by tiles_x
+    int lean;                          // the LEAN instantiations may be used
 };
 
 // Receiver-adjoint injection (fwi/adjoint_jax_kernel.py:201-206; duplicate
@@ -980,9 +998,13 @@ struct K4Src {
 
 // strain-adjoint sweep: steps 0-T..2-T of fwi/adjoint_jax_kernel.py:201-323;
 // produces the final U_a, V_a of the step and the source-wavelet adjoint
-template <typename T, int FD, int TZ>
+template <typename T, int FD, int TZ, bool LEAN>
 __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
 {
+    if constexpr (LEAN) {
+        // no source-wavelet adjoint
+        a.src_adj = nullptr;
+    }
     const Geo<T> &g = a.g;
     int bx, s;
     logical_block(a.dgx, bx, s);
@@ -1310,9 +1332,16 @@ struct K5Src {
 
 // velocity-adjoint sweep with the imaging condition fused: steps 2-T (V),
 // 1-T (U) and the constitutive transposes (fwi/adjoint_jax_kernel.py:325-464)
-template <typename T, int FD, int TZ>
+template <typename T, int FD, int TZ, bool LEAN>
 __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
 {
+    if constexpr (LEAN) {
+        // exact imaging of the Vs gradient only; no snapshots or energy
+        a.snap = nullptr;
+        a.aen = nullptr;
+        a.imaging = 2;
+        a.gparams = 1;
+    }
     const Geo<T> &g = a.g;
     int bx, s;
     logical_block(a.dgx, bx, s);

@@ -71,6 +71,8 @@ struct stfwi_plan {
     // shots are split into `groups` contiguous groups whose time loops run on
     // their own streams, so one group's launches fill the other's tails
     int groups = 1;
+    // the time-step kernels' LEAN instantiations may be used (on by default)
+    int lean = 1;
     hipStream_t gstream[STFWI_MAX_GROUPS] = {};
     hipEvent_t ev_fork = nullptr, ev_join[STFWI_MAX_GROUPS] = {};
     // graph replay of the time loops: the forward and the adjoint are each
@@ -417,10 +419,17 @@ static void launch_fwd(int which, int tz, FwdArgs<T> a, int rblk, int S, hipStre
     a.dgx = make_udiv((unsigned)(a.ntiles + rblk));
     a.dtx = make_udiv((unsigned)a.tiles_x);
     dim3 grid(a.ntiles + rblk, 1, S), blk(TILE_X, BLOCK_Y);
-#define STFWI_LF(TZV)                                                                         \
-    do {                                                                                      \
-        if (which == 0) hipLaunchKernelGGL((k_fwd_vel<T, FD, TZV>), grid, blk, 0, st, a, it);    \
-        else hipLaunchKernelGGL((k_fwd_strain<T, FD, TZV>), grid, blk, 0, st, a, it);            \
+    // the LEAN instantiations fix what these arguments already hold
+    const bool lean = a.lean && a.hist2_mode == 0 && (which == 0 || (!a.illum && !a.snap));
+#define STFWI_LF(TZV)                                                                              \
+    do {                                                                                           \
+        if (which == 0) {                                                                          \
+            if (lean) hipLaunchKernelGGL((k_fwd_vel<T, FD, TZV, true>), grid, blk, 0, st, a, it);     \
+            else hipLaunchKernelGGL((k_fwd_vel<T, FD, TZV, false>), grid, blk, 0, st, a, it);         \
+        } else {                                                                                   \
+            if (lean) hipLaunchKernelGGL((k_fwd_strain<T, FD, TZV, true>), grid, blk, 0, st, a, it);  \
+            else hipLaunchKernelGGL((k_fwd_strain<T, FD, TZV, false>), grid, blk, 0, st, a, it);      \
+        }                                                                                          \
     } while (0)
     if constexpr (std::is_same<T, float>::value) {
         if (tz == 32) STFWI_LF(32);
@@ -442,10 +451,18 @@ static void launch_adj(int which, int tz, AdjArgs<T> a, int S, hipStream_t st, i
     a.dgx = make_udiv((unsigned)a.ntiles);
     a.dtx = make_udiv((unsigned)a.tiles_x);
     dim3 grid(a.ntiles, 1, S), blk(TILE_X, BLOCK_Y);
-#define STFWI_LA(TZV)                                                                          \
-    do {                                                                                       \
-        if (which == 2) hipLaunchKernelGGL((k_adj_strain<T, FD, TZV>), grid, blk, 0, st, a, it);  \
-        else hipLaunchKernelGGL((k_adj_vel<T, FD, TZV>), grid, blk, 0, st, a, it);                \
+    // the LEAN instantiations fix what these arguments already hold
+    const bool lean = a.lean && (which == 2 ? !a.src_adj
+                                            : (!a.snap && !a.aen && a.imaging == 2 && a.gparams == 1));
+#define STFWI_LA(TZV)                                                                              \
+    do {                                                                                           \
+        if (which == 2) {                                                                          \
+            if (lean) hipLaunchKernelGGL((k_adj_strain<T, FD, TZV, true>), grid, blk, 0, st, a, it);  \
+            else hipLaunchKernelGGL((k_adj_strain<T, FD, TZV, false>), grid, blk, 0, st, a, it);      \
+        } else {                                                                                   \
+            if (lean) hipLaunchKernelGGL((k_adj_vel<T, FD, TZV, true>), grid, blk, 0, st, a, it);     \
+            else hipLaunchKernelGGL((k_adj_vel<T, FD, TZV, false>), grid, blk, 0, st, a, it);         \
+        }                                                                                          \
     } while (0)
     if constexpr (std::is_same<T, float>::value) {
         if (tz == 32) STFWI_LA(32);
@@ -518,6 +535,7 @@ static FwdArgs<T> fwd_args(stfwi_plan *p, void *rec, int rec_mask, void *illum, 
     a.illum = (T *)illum; a.illum_comp = illum_comp;
     a.snap = (T *)snap; a.snap_mask = snap_mask; a.nsnap = popcount5(snap_mask);
     a.tiles_x = (p->nx + TILE_X - 1) / TILE_X;
+    a.lean = p->lean;
     return a;
 }
 
@@ -551,6 +569,7 @@ static AdjArgs<T> adj_args(stfwi_plan *p, const void *rec_adj, void *src_adj, vo
     a.snap = (T *)snap; a.snap_mask = snap_mask; a.nsnap = popcount5(snap_mask);
     a.aen = (T *)p->aen; a.aen_comp = p->aen_comp;
     a.tiles_x = (p->nx + TILE_X - 1) / TILE_X;
+    a.lean = p->lean;
     return a;
 }
 
@@ -1005,6 +1024,13 @@ extern "C" int stfwi_set_adjoint_energy(stfwi_plan *p, void *energy, int comp)
     if (!p || comp < 0 || comp > 4) return fail(STFWI_ERR_INVALID, "bad argument (comp=%d)", comp);
     p->aen = energy;
     p->aen_comp = comp;
+    return 0;
+}
+
+extern "C" int stfwi_set_lean(stfwi_plan *p, int enable)
+{
+    if (!p) return fail(STFWI_ERR_INVALID, "null plan");
+    p->lean = enable ? 1 : 0;
     return 0;
 }
 

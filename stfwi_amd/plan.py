@@ -254,6 +254,12 @@ class Plan:
         """Run the shots as `groups` concurrent stream groups (1..4)."""
         check(lib().stfwi_set_groups(self._h, int(groups)))
 
+    def set_lean(self, enable=True):
+        """Allow (the default) or forbid the time-step kernels' specialised
+        instantiations for runs without optional outputs; the results are
+        the same bit for bit either way."""
+        check(lib().stfwi_set_lean(self._h, int(bool(enable))))
+
     def set_tiles(self, tz):
         arr = (C.c_int * 4)(*[int(t) for t in tz])
         check(lib().stfwi_set_tiles(self._h, arr))

@@ -349,3 +349,57 @@ def test_gpu_residual_unfiltered_ragged_f64(ntap):
         assert rel_l2(radj[k, 1, :nt, :nr], cot) < TOL_F64, k
         assert not radj[k, 1, nt:].any() and not radj[k, 1, :, nr:].any(), k
         assert not radj[k, [0, 2, 3]].any(), k
+
+
+@pytest.mark.parametrize("fd,fs,st", [(8, True, "moment"), (4, False, "force"), (2, True, "force")])
+def test_gpu_vs_only_exact_gradient_matches_oracle_f64(fd, fs, st):
+    """The exact Vs-only gradient (the path the specialised velocity-adjoint
+    kernel serves) against the oracle's exact chain."""
+    m, sx, sz, rx, rz, wav, obs = _grad_case(fd, fs, st, seed=5)
+    print(f"case fd={fd} fs={fs} src={st}", flush=True)
+    p = _plan(m, fd, fs, torch.float64, nrec=rx.size, imaging="exact", grad="vs", hist=True)
+    p.set_model(m["Vs"], m["Vp"], m["rho"])
+    p.set_shots([sx], [sz], st, m["nt"], rx, rz, wav)
+    rec = p.forward()["rec"]
+    loss, radj = p.residual(rec, torch.as_tensor(obs[None]), 2, None, 0, 1e15, 1)
+    p.adjoint(radj, want_src=False)
+    gg = p.finalize(m["Vs"], m["Vp"], m["rho"]).cpu().numpy()
+    g = oracle.Geom(m["nz"], m["nx"], m["pad"], fs, fd, m["nt"], m["dx"], m["dz"], m["dt"], m["fc"])
+    rec_o, t = oracle.forward(g, m["Vs"], m["Vp"], m["rho"], sx, sz, wav, rx, rz, st, np.float64,
+                              ("e", "dxu0", "s"))
+    _, cot = osig.residual_prep(rec_o[2], obs, None, 0, 1e15, 1)
+    ra = np.zeros_like(rec_o)
+    ra[2] = cot
+    a = oracle.adjoint(g, m["Vs"], m["Vp"], m["rho"], sx, sz, ra, rx, rz, st, wav, "exact", t)
+    go = oracle.exact_chain(g, a["g"], m["Vs"], m["Vp"], m["rho"])
+    assert rel_l2(gg[0], go[0]) < TOL_F64
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_gpu_lean_kernels_agree_bitwise(dtype):
+    """The time-step kernels' specialised instantiations (no optional
+    outputs; exact imaging of Vs only) give the same records, source
+    adjoints and gradients bit for bit as the general ones, in stream groups
+    and with uneven record lengths; asking for the source adjoint (general
+    strain-adjoint kernel) leaves the gradient unchanged."""
+    m, rx, rz, wav = _big_case()
+    rng = np.random.default_rng(8)
+    sx, sz, st = [30, 77, 150], [3, 4, 2], ["moment", "force", "moment"]
+    obs = torch.as_tensor(rng.standard_normal((3, m["nt"], rx.size)) * 1e-9, dtype=dtype)
+    p = _plan(m, 8, True, dtype, nshots=3, nrec=rx.size, imaging="exact", grad="vs", hist=True)
+    p.set_model(m["Vs"], m["Vp"], m["rho"])
+    p.set_shots(sx, sz, st, [m["nt"], m["nt"] - 23, m["nt"] - 5], rx, rz, wav)
+    p.set_groups(2)
+    outs = []
+    for lean in (True, False):
+        p.set_lean(lean)
+        rec = p.forward()["rec"]
+        _, radj = p.residual(rec, obs, 2, None, 0, 1e15, 3)
+        p.adjoint(radj, want_src=False)
+        g1 = p.finalize(m["Vs"], m["Vp"], m["rho"]).clone()
+        sa = p.adjoint(radj)["src_adj"].clone()
+        g2 = p.finalize(m["Vs"], m["Vp"], m["rho"]).clone()
+        assert torch.equal(g1, g2)
+        outs.append((rec.clone(), sa, g1))
+    for x, y in zip(outs[0], outs[1]):
+        assert torch.equal(x, y)
