@@ -215,8 +215,9 @@ int stfwi_time_kernel(stfwi_plan *plan, int which, int reps, void *stream, doubl
 /* Tile-shape autotuner (no reference counterpart): for each of the four
  * time-step kernels time `reps` launches at every compiled tile height
  * (4/8/16/32 rows of 64 columns in FP32, 4/8 in FP64) on the plan's shots,
- * in three interleaved rounds of which the fastest counts, and keep the
- * fastest height.  tz_out (4 ints) and us_out (4 x 4 average
+ * each repetition launching every shot group (stfwi_set_groups) on its own
+ * stream as the time loops do, in three interleaved rounds of which the
+ * fastest counts, and keep the fastest height.  tz_out (4 ints) and us_out (4 x 4 average
  * microseconds, 0 where not compiled) may be NULL.  Host call, synchronous; overwrites the
  * wavefield state, so run it before stfwi_forward. */
 int stfwi_autotune(stfwi_plan *plan, int reps, void *stream, int *tz_out, double *us_out);

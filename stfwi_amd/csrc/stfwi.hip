@@ -927,12 +927,18 @@ extern "C" int stfwi_residual(stfwi_plan *p, const void *pred, const void *obs, 
 
 // ---------------------------------------------------------------------------
 // diagnostics: time `reps` launches of one time-step kernel for all shots of
-// the plan at a mid-record step, bracketed by HIP events on `stream`
+// the plan at a mid-record step, bracketed by HIP events on `stream`.  With
+// grouped = true each repetition launches every shot group on its own
+// stream, as the time loops do (the autotuner times candidates this way);
+// otherwise one launch covers all shots.
 // ---------------------------------------------------------------------------
 template <typename T, int FD>
-static int time_kernel_t(stfwi_plan *p, int which, int tz, int reps, hipStream_t st, double *ms)
+static int time_kernel_t(stfwi_plan *p, int which, int tz, int reps, hipStream_t st, double *ms, bool grouped)
 {
     const int it = p->nt_run / 2;
+    int g0[STFWI_MAX_GROUPS + 1] = {0, p->n_shots};
+    const int G = grouped ? group_bounds(p, g0) : 1;
+    hipStream_t gs[STFWI_MAX_GROUPS];
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -940,7 +946,13 @@ static int time_kernel_t(stfwi_plan *p, int which, int tz, int reps, hipStream_t
         FwdArgs<T> a = fwd_args<T>(p, nullptr, 0, nullptr, 0, nullptr, 0);
         if (p->ck) a.hist_t0 = it - 1;    // frames 0..2 of a checkpoint segment
         HIP_TRY(hipEventRecord(e0, st));
-        for (int r = 0; r < reps; ++r) launch_fwd<T, FD>(which, tz, a, 0, p->n_shots, st, it);
+        HIP_TRY(fork_groups(p, G, st, gs));
+        for (int r = 0; r < reps; ++r)
+            for (int k = 0; k < G; ++k) {
+                a.shot0 = g0[k];
+                launch_fwd<T, FD>(which, tz, a, 0, g0[k + 1] - g0[k], gs[k], it);
+            }
+        HIP_TRY(join_groups(p, G, st));
         HIP_TRY(hipEventRecord(e1, st));
     } else {
         AdjArgs<T> a = adj_args<T>(p, nullptr, nullptr, nullptr, 0);
@@ -948,7 +960,13 @@ static int time_kernel_t(stfwi_plan *p, int which, int tz, int reps, hipStream_t
         a.psix_o = (const T *)p->psix_a[0]; a.psiz_o = (const T *)p->psiz_a[0];
         a.psix_n = (T *)p->psix_a[1]; a.psiz_n = (T *)p->psiz_a[1];
         HIP_TRY(hipEventRecord(e0, st));
-        for (int r = 0; r < reps; ++r) launch_adj<T, FD>(which, tz, a, p->n_shots, st, it);
+        HIP_TRY(fork_groups(p, G, st, gs));
+        for (int r = 0; r < reps; ++r)
+            for (int k = 0; k < G; ++k) {
+                a.shot0 = g0[k];
+                launch_adj<T, FD>(which, tz, a, g0[k + 1] - g0[k], gs[k], it);
+            }
+        HIP_TRY(join_groups(p, G, st));
         HIP_TRY(hipEventRecord(e1, st));
     }
     HIP_TRY(hipGetLastError());
@@ -961,16 +979,17 @@ static int time_kernel_t(stfwi_plan *p, int which, int tz, int reps, hipStream_t
     return 0;
 }
 
-static int time_kernel(stfwi_plan *p, int which, int tz, int reps, hipStream_t st, double *ms)
+static int time_kernel(stfwi_plan *p, int which, int tz, int reps, hipStream_t st, double *ms,
+                       bool grouped = false)
 {
     const int fd = p->cfg.fd_order;
     if (p->cfg.dtype == STFWI_F64)
-        return fd == 8 ? time_kernel_t<double, 8>(p, which, tz, reps, st, ms)
-             : fd == 4 ? time_kernel_t<double, 4>(p, which, tz, reps, st, ms)
-                       : time_kernel_t<double, 2>(p, which, tz, reps, st, ms);
-    return fd == 8 ? time_kernel_t<float, 8>(p, which, tz, reps, st, ms)
-         : fd == 4 ? time_kernel_t<float, 4>(p, which, tz, reps, st, ms)
-                   : time_kernel_t<float, 2>(p, which, tz, reps, st, ms);
+        return fd == 8 ? time_kernel_t<double, 8>(p, which, tz, reps, st, ms, grouped)
+             : fd == 4 ? time_kernel_t<double, 4>(p, which, tz, reps, st, ms, grouped)
+                       : time_kernel_t<double, 2>(p, which, tz, reps, st, ms, grouped);
+    return fd == 8 ? time_kernel_t<float, 8>(p, which, tz, reps, st, ms, grouped)
+         : fd == 4 ? time_kernel_t<float, 4>(p, which, tz, reps, st, ms, grouped)
+                   : time_kernel_t<float, 2>(p, which, tz, reps, st, ms, grouped);
 }
 
 extern "C" int stfwi_time_kernel(stfwi_plan *p, int which, int reps, void *stream, double *ms)
@@ -988,15 +1007,17 @@ extern "C" int stfwi_autotune(stfwi_plan *p, int reps, void *stream, int *tz_out
     const bool f64 = p->cfg.dtype == STFWI_F64;
     for (int which = 0; which < 4; ++which) {
         // three interleaved rounds, the fastest round of each candidate: a
-        // transient slowdown during one round does not decide the choice
+        // transient slowdown during one round does not decide the choice.
+        // Each candidate runs as the time loops run it: the plan's shot
+        // groups launched concurrently on their streams.
         double t[4] = {1e300, 1e300, 1e300, 1e300};
         for (int round = 0; round < 3; ++round)
             for (int ci = 0; ci < 4; ++ci) {
                 const int tz = cand[ci];
                 if (f64 ? !tz_supported<double>(tz) : !tz_supported<float>(tz)) continue;
                 double ms = 0;
-                int rc = time_kernel(p, which, tz, 2, (hipStream_t)stream, &ms);   // warm-up
-                if (!rc) rc = time_kernel(p, which, tz, reps, (hipStream_t)stream, &ms);
+                int rc = time_kernel(p, which, tz, 2, (hipStream_t)stream, &ms, true);   // warm-up
+                if (!rc) rc = time_kernel(p, which, tz, reps, (hipStream_t)stream, &ms, true);
                 if (rc) return rc;
                 t[ci] = std::min(t[ci], ms);
             }
