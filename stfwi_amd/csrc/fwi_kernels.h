@@ -1355,7 +1355,10 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
     const int nz = g.nz, nx = g.nx, ld = g.ld;
     const int tz = (int)udiv((unsigned)bx, a.dtx), tx = bx - tz * a.tiles_x;
     const int r0 = tz * TZ, c0 = tx * TILE_X;
-    const int lx = threadIdx.x, ty = threadIdx.y, tid = ty * TILE_X + lx;
+    // the row index is uniform across a wavefront (64 x 4 threads): made
+    // scalar here, the z-dependent masks and C-PML row lookups of this kernel
+    // become scalar work (measured faster for this kernel only)
+    const int lx = threadIdx.x, ty = __builtin_amdgcn_readfirstlane(threadIdx.y), tid = ty * TILE_X + lx;
     T *st = a.state + s * a.sstride + g.origin;
     const T *Ua = st, *Va = st + g.fstride;
     T *exa = st + 2 * g.fstride, *eza = st + 3 * g.fstride, *esa = st + 4 * g.fstride;
