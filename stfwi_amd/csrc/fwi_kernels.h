@@ -1198,6 +1198,22 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
     __syncthreads();
     auto body = [&](auto tagx, auto tagz) {
         constexpr bool IX = decltype(tagx)::value, IZ = decltype(tagz)::value;
+        // the column (x) term masks as per-lane weights, set once per tile:
+        // a masked term adds 0 * value (the staged values are finite), an
+        // unmasked one its coefficient; along an interior axis they fold to
+        // the constant coefficients
+        T wzu[NK], wvx1[NK], wvx2[NK], wzv[NK], wux1[NK], wux2[NK];
+#pragma unroll
+        for (int k = 1; k <= NK; ++k) {
+            const T ck = (T)fdc<FD>(k);
+            const int i1 = c - k + 1, i2 = c + k;
+            wzu[k - 1] = CX(c >= 1) ? ck : T(0);
+            wvx1[k - 1] = CX(c - k >= k - 1 && c - k <= nx - 1 - k) ? ck : T(0);
+            wvx2[k - 1] = CX(c + k - 1 >= k - 1 && c + k - 1 <= nx - 1 - k) ? -ck : T(0);
+            wzv[k - 1] = CX(c <= nx - 2) ? ck : T(0);
+            wux1[k - 1] = CX(k == 1 ? (i1 >= 1 && i1 <= nx - 1) : (i1 >= k && i1 <= nx - k)) ? ck : T(0);
+            wux2[k - 1] = CX(k == 1 ? (i2 >= 1 && i2 <= nx - 1) : (i2 >= k && i2 <= nx - k)) ? -ck : T(0);
+        }
 #pragma unroll
         for (int rr = 0; rr < RPT; ++rr) {
             const int lz = ty + BLOCK_Y * rr, r = r0 + lz;
@@ -1207,35 +1223,27 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
             T ua = ua0[rr], va = va0[rr];
 #pragma unroll
             for (int k = 1; k <= NK; ++k) {
-                const T ck = (T)fdc<FD>(k);
                 // (D+z)^T of Dz_U_a, columns [1, nx-1]; term range [k-1, nz-1-k]
-                if (CX(c >= 1)) {
-                    if (CZ(r - k >= k - 1 && r - k <= nz - 1 - k)) ua += ck * szU[zc - k][lx];
-                    if (CZ(r + k - 1 >= k - 1 && r + k - 1 <= nz - 1 - k)) ua += -ck * szU[zc + k - 1][lx];
-                }
+                if (CZ(r - k >= k - 1 && r - k <= nz - 1 - k)) ua += wzu[k - 1] * szU[zc - k][lx];
+                if (CZ(r + k - 1 >= k - 1 && r + k - 1 <= nz - 1 - k)) ua += -wzu[k - 1] * szU[zc + k - 1][lx];
                 // (D+x)^T of Dx_V_a, rows [0, nz-2]
                 if (CZ(r <= nz - 2)) {
-                    if (CX(c - k >= k - 1 && c - k <= nx - 1 - k)) va += ck * sxV[lz][xc - k];
-                    if (CX(c + k - 1 >= k - 1 && c + k - 1 <= nx - 1 - k)) va += -ck * sxV[lz][xc + k - 1];
+                    va += wvx1[k - 1] * sxV[lz][xc - k];
+                    va += wvx2[k - 1] * sxV[lz][xc + k - 1];
                 }
             }
 #pragma unroll
             for (int k = 1; k <= NK; ++k) {
-                const T ck = (T)fdc<FD>(k);
                 // (D-z)^T of Dz_V_a, columns [0, nx-2]; term 1 rows [1, nz-1], k: [k, nz-k]
-                if (CX(c <= nx - 2)) {
-                    const int i1 = r - k + 1, i2 = r + k;
-                    if (CZ(k == 1 ? (i1 >= 1 && i1 <= nz - 1) : (i1 >= k && i1 <= nz - k))) va += ck * szV[zc - k + 1][lx];
-                    if (CZ(k == 1 ? (i2 >= 1 && i2 <= nz - 1) : (i2 >= k && i2 <= nz - k))) va += -ck * szV[zc + k][lx];
-                }
+                const int i1 = r - k + 1, i2 = r + k;
+                if (CZ(k == 1 ? (i1 >= 1 && i1 <= nz - 1) : (i1 >= k && i1 <= nz - k))) va += wzv[k - 1] * szV[zc - k + 1][lx];
+                if (CZ(k == 1 ? (i2 >= 1 && i2 <= nz - 1) : (i2 >= k && i2 <= nz - k))) va += -wzv[k - 1] * szV[zc + k][lx];
             }
 #pragma unroll
             for (int k = 1; k <= NK; ++k) {
-                const T ck = (T)fdc<FD>(k);
                 // (D-x)^T of Dx_U_a, all rows; term 1 cols [1, nx-1], k: [k, nx-k]
-                const int i1 = c - k + 1, i2 = c + k;
-                if (CX(k == 1 ? (i1 >= 1 && i1 <= nx - 1) : (i1 >= k && i1 <= nx - k))) ua += ck * sxU[lz][xc - k + 1];
-                if (CX(k == 1 ? (i2 >= 1 && i2 <= nx - 1) : (i2 >= k && i2 <= nx - k))) ua += -ck * sxU[lz][xc + k];
+                ua += wux1[k - 1] * sxU[lz][xc - k + 1];
+                ua += wux2[k - 1] * sxU[lz][xc + k];
             }
             stg(Ua, q, ua);
             stg(Va, q, va);
@@ -1567,6 +1575,19 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
     __syncthreads();
     auto body = [&](auto tagx, auto tagz) {
         constexpr bool IX = decltype(tagx)::value, IZ = decltype(tagz)::value;
+        // the column (x) term masks as per-lane weights, set once per tile
+        // (a masked term adds 0 * a finite staged value)
+        T wax1[NK], wax2[NK], wzi[NK], wmx1[NK], wmx2[NK];
+#pragma unroll
+        for (int k = 1; k <= NK; ++k) {
+            const T ck = (T)fdc<FD>(k);
+            const int i1 = c - k + 1, i2 = c + k;
+            wax1[k - 1] = CX(c - k >= k - 1 && c - k <= nx - 1 - k) ? ck : T(0);
+            wax2[k - 1] = CX(c + k - 1 >= k - 1 && c + k - 1 <= nx - 1 - k) ? -ck : T(0);
+            wzi[k - 1] = CX(c >= 1 && c <= nx - 2) ? ck : T(0);
+            wmx1[k - 1] = CX(k == 1 ? (i1 >= 1 && i1 <= nx - 2) : (i1 >= k && i1 <= nx - k)) ? ck : T(0);
+            wmx2[k - 1] = CX(k == 1 ? (i2 >= 1 && i2 <= nx - 2) : (i2 >= k && i2 <= nx - k)) ? -ck : T(0);
+        }
 #pragma unroll
         for (int rr = 0; rr < RPT; ++rr) {
             const int lz = ty + BLOCK_Y * rr, r = r0 + lz;
@@ -1576,37 +1597,29 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
             T Ax = 0, Az = 0, Mes = 0;
 #pragma unroll
             for (int k = 1; k <= NK; ++k) {
-                const T ck = (T)fdc<FD>(k);
                 // (D+x)^T of Dx_(B U)_a (= L2M_ex_a = L_ez_a), rows [0, nz-2]
                 if (CZ(r <= nz - 2)) {
-                    if (CX(c - k >= k - 1 && c - k <= nx - 1 - k)) Ax += ck * sxBU[lz][xc - k];
-                    if (CX(c + k - 1 >= k - 1 && c + k - 1 <= nx - 1 - k)) Ax += -ck * sxBU[lz][xc + k - 1];
+                    Ax += wax1[k - 1] * sxBU[lz][xc - k];
+                    Ax += wax2[k - 1] * sxBU[lz][xc + k - 1];
                 }
                 // (D+z)^T of Dz_(B V)_a (= L2M_ez_a = L_ex_a), columns [1, nx-2]
-                if (CX(c >= 1 && c <= nx - 2)) {
-                    if (CZ(r - k >= k - 1 && r - k <= nz - 1 - k)) Az += ck * szBV[zc - k][lx];
-                    if (CZ(r + k - 1 >= k - 1 && r + k - 1 <= nz - 1 - k)) Az += -ck * szBV[zc + k - 1][lx];
-                }
+                if (CZ(r - k >= k - 1 && r - k <= nz - 1 - k)) Az += wzi[k - 1] * szBV[zc - k][lx];
+                if (CZ(r + k - 1 >= k - 1 && r + k - 1 <= nz - 1 - k)) Az += -wzi[k - 1] * szBV[zc + k - 1][lx];
             }
 #pragma unroll
             for (int k = 1; k <= NK; ++k) {
-                const T ck = (T)fdc<FD>(k);
                 // (D-x)^T of Dx_M_es_a, rows [0, nz-2]; term 1 cols [1, nx-2], k: [k, nx-k]
                 if (CZ(r <= nz - 2)) {
-                    const int i1 = c - k + 1, i2 = c + k;
-                    if (CX(k == 1 ? (i1 >= 1 && i1 <= nx - 2) : (i1 >= k && i1 <= nx - k))) Mes += ck * sxMes[lz][xc - k + 1];
-                    if (CX(k == 1 ? (i2 >= 1 && i2 <= nx - 2) : (i2 >= k && i2 <= nx - k))) Mes += -ck * sxMes[lz][xc + k];
+                    Mes += wmx1[k - 1] * sxMes[lz][xc - k + 1];
+                    Mes += wmx2[k - 1] * sxMes[lz][xc + k];
                 }
             }
 #pragma unroll
             for (int k = 1; k <= NK; ++k) {
-                const T ck = (T)fdc<FD>(k);
                 // (D-z)^T of Dz_M_es_a, columns [1, nx-2]; term 1 rows [1, nz-2], k: [k, nz-k]
-                if (CX(c >= 1 && c <= nx - 2)) {
-                    const int i1 = r - k + 1, i2 = r + k;
-                    if (CZ(k == 1 ? (i1 >= 1 && i1 <= nz - 2) : (i1 >= k && i1 <= nz - k))) Mes += ck * szMes[zc - k + 1][lx];
-                    if (CZ(k == 1 ? (i2 >= 1 && i2 <= nz - 2) : (i2 >= k && i2 <= nz - k))) Mes += -ck * szMes[zc + k][lx];
-                }
+                const int i1 = r - k + 1, i2 = r + k;
+                if (CZ(k == 1 ? (i1 >= 1 && i1 <= nz - 2) : (i1 >= k && i1 <= nz - k))) Mes += wzi[k - 1] * szMes[zc - k + 1][lx];
+                if (CZ(k == 1 ? (i2 >= 1 && i2 <= nz - 2) : (i2 >= k && i2 <= nz - k))) Mes += -wzi[k - 1] * szMes[zc + k][lx];
             }
             if (!IZ && g.fs && r == 0 && c >= 1 && c <= nx - 2) Mes += T(2 * fdc<FD>(1)) * szMes[zc][lx];
 
