@@ -1221,23 +1221,27 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
             const int xc = lx + 4, zc = lz + 4;
             const int q = r * ld + c;
             T ua = ua0[rr], va = va0[rr];
+            // the row (z) term masks as selects of the weights: no branches
+            const bool rz = CZ(r <= nz - 2);
 #pragma unroll
             for (int k = 1; k <= NK; ++k) {
                 // (D+z)^T of Dz_U_a, columns [1, nx-1]; term range [k-1, nz-1-k]
-                if (CZ(r - k >= k - 1 && r - k <= nz - 1 - k)) ua += wzu[k - 1] * szU[zc - k][lx];
-                if (CZ(r + k - 1 >= k - 1 && r + k - 1 <= nz - 1 - k)) ua += -wzu[k - 1] * szU[zc + k - 1][lx];
+                const T u1 = CZ(r - k >= k - 1 && r - k <= nz - 1 - k) ? wzu[k - 1] : T(0);
+                const T u2 = CZ(r + k - 1 >= k - 1 && r + k - 1 <= nz - 1 - k) ? -wzu[k - 1] : T(0);
+                ua += u1 * szU[zc - k][lx];
+                ua += u2 * szU[zc + k - 1][lx];
                 // (D+x)^T of Dx_V_a, rows [0, nz-2]
-                if (CZ(r <= nz - 2)) {
-                    va += wvx1[k - 1] * sxV[lz][xc - k];
-                    va += wvx2[k - 1] * sxV[lz][xc + k - 1];
-                }
+                va += (rz ? wvx1[k - 1] : T(0)) * sxV[lz][xc - k];
+                va += (rz ? wvx2[k - 1] : T(0)) * sxV[lz][xc + k - 1];
             }
 #pragma unroll
             for (int k = 1; k <= NK; ++k) {
                 // (D-z)^T of Dz_V_a, columns [0, nx-2]; term 1 rows [1, nz-1], k: [k, nz-k]
                 const int i1 = r - k + 1, i2 = r + k;
-                if (CZ(k == 1 ? (i1 >= 1 && i1 <= nz - 1) : (i1 >= k && i1 <= nz - k))) va += wzv[k - 1] * szV[zc - k + 1][lx];
-                if (CZ(k == 1 ? (i2 >= 1 && i2 <= nz - 1) : (i2 >= k && i2 <= nz - k))) va += -wzv[k - 1] * szV[zc + k][lx];
+                const T v1 = CZ(k == 1 ? (i1 >= 1 && i1 <= nz - 1) : (i1 >= k && i1 <= nz - k)) ? wzv[k - 1] : T(0);
+                const T v2 = CZ(k == 1 ? (i2 >= 1 && i2 <= nz - 1) : (i2 >= k && i2 <= nz - k)) ? -wzv[k - 1] : T(0);
+                va += v1 * szV[zc - k + 1][lx];
+                va += v2 * szV[zc + k][lx];
             }
 #pragma unroll
             for (int k = 1; k <= NK; ++k) {
