@@ -584,15 +584,14 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
                     D2 += ck * (sP2[lz][xc + k] - sP2[lz][xc - k + 1]);
                 }
                 // D-z of 2 mu exz, term 1 rows [1, nz-2], term k rows [k, nz-k]
-                if (CZ(k == 1 ? (r >= 1 && r <= nz - 2) : (r >= k && r <= nz - k)))
-                    D3 += ck * (sP3z[zc + k - 1][lx] - sP3z[zc - k][lx]);
+                D3 += (CZ(k == 1 ? (r >= 1 && r <= nz - 2) : (r >= k && r <= nz - k)) ? ck : T(0)) *
+                      (sP3z[zc + k - 1][lx] - sP3z[zc - k][lx]);
                 // D-x of 2 mu exz, term 1 cols [1, nx-2], term k cols [k, nx-k]
                 if (CX(k == 1 || (c >= k && c <= nx - k))) D4 += ck * (sP3x[lz][xc + k - 1] - sP3x[lz][xc - k]);
                 // D+z, term range [k-1, nz-1-k]
-                if (CZ(r >= k - 1 && r <= nz - 1 - k)) {
-                    D5 += ck * (sP4[zc + k][lx] - sP4[zc - k + 1][lx]);
-                    D6 += ck * (sP5[zc + k][lx] - sP5[zc - k + 1][lx]);
-                }
+                const T w56 = CZ(r >= k - 1 && r <= nz - 1 - k) ? ck : T(0);
+                D5 += w56 * (sP4[zc + k][lx] - sP4[zc - k + 1][lx]);
+                D6 += w56 * (sP5[zc + k][lx] - sP5[zc - k + 1][lx]);
             }
             if (!IZ && g.fs && r == 0) D3 = T(2 * fdc<FD>(1)) * sP3z[zc][lx];   // image of sigma_xz at the surface
             if (!IX) {
@@ -1599,31 +1598,33 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
             const int xc = lx + 4, zc = lz + 4;
             const int q = r * ld + c;
             T Ax = 0, Az = 0, Mes = 0;
+            // the row (z) term masks as selects of the weights: no branches
+            const bool rz = CZ(r <= nz - 2);
 #pragma unroll
             for (int k = 1; k <= NK; ++k) {
                 // (D+x)^T of Dx_(B U)_a (= L2M_ex_a = L_ez_a), rows [0, nz-2]
-                if (CZ(r <= nz - 2)) {
-                    Ax += wax1[k - 1] * sxBU[lz][xc - k];
-                    Ax += wax2[k - 1] * sxBU[lz][xc + k - 1];
-                }
+                Ax += (rz ? wax1[k - 1] : T(0)) * sxBU[lz][xc - k];
+                Ax += (rz ? wax2[k - 1] : T(0)) * sxBU[lz][xc + k - 1];
                 // (D+z)^T of Dz_(B V)_a (= L2M_ez_a = L_ex_a), columns [1, nx-2]
-                if (CZ(r - k >= k - 1 && r - k <= nz - 1 - k)) Az += wzi[k - 1] * szBV[zc - k][lx];
-                if (CZ(r + k - 1 >= k - 1 && r + k - 1 <= nz - 1 - k)) Az += -wzi[k - 1] * szBV[zc + k - 1][lx];
+                const T z1 = CZ(r - k >= k - 1 && r - k <= nz - 1 - k) ? wzi[k - 1] : T(0);
+                const T z2 = CZ(r + k - 1 >= k - 1 && r + k - 1 <= nz - 1 - k) ? -wzi[k - 1] : T(0);
+                Az += z1 * szBV[zc - k][lx];
+                Az += z2 * szBV[zc + k - 1][lx];
             }
 #pragma unroll
             for (int k = 1; k <= NK; ++k) {
                 // (D-x)^T of Dx_M_es_a, rows [0, nz-2]; term 1 cols [1, nx-2], k: [k, nx-k]
-                if (CZ(r <= nz - 2)) {
-                    Mes += wmx1[k - 1] * sxMes[lz][xc - k + 1];
-                    Mes += wmx2[k - 1] * sxMes[lz][xc + k];
-                }
+                Mes += (rz ? wmx1[k - 1] : T(0)) * sxMes[lz][xc - k + 1];
+                Mes += (rz ? wmx2[k - 1] : T(0)) * sxMes[lz][xc + k];
             }
 #pragma unroll
             for (int k = 1; k <= NK; ++k) {
                 // (D-z)^T of Dz_M_es_a, columns [1, nx-2]; term 1 rows [1, nz-2], k: [k, nz-k]
                 const int i1 = r - k + 1, i2 = r + k;
-                if (CZ(k == 1 ? (i1 >= 1 && i1 <= nz - 2) : (i1 >= k && i1 <= nz - k))) Mes += wzi[k - 1] * szMes[zc - k + 1][lx];
-                if (CZ(k == 1 ? (i2 >= 1 && i2 <= nz - 2) : (i2 >= k && i2 <= nz - k))) Mes += -wzi[k - 1] * szMes[zc + k][lx];
+                const T m1 = CZ(k == 1 ? (i1 >= 1 && i1 <= nz - 2) : (i1 >= k && i1 <= nz - k)) ? wzi[k - 1] : T(0);
+                const T m2 = CZ(k == 1 ? (i2 >= 1 && i2 <= nz - 2) : (i2 >= k && i2 <= nz - k)) ? -wzi[k - 1] : T(0);
+                Mes += m1 * szMes[zc - k + 1][lx];
+                Mes += m2 * szMes[zc + k][lx];
             }
             if (!IZ && g.fs && r == 0 && c >= 1 && c <= nx - 2) Mes += T(2 * fdc<FD>(1)) * szMes[zc][lx];
 
