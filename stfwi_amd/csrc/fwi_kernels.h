@@ -778,8 +778,8 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
                 } else {
 #pragma unroll
                     for (int k = 1; k <= NK; ++k)
-                        if (CZ(k == 1 ? r >= 1 : (r >= k && r <= nz - k)))
-                            dzv += (T)fdc<FD>(k) * (sV[zc + k - 1][xc] - sV[zc - k][xc]);
+                        dzv += (CZ(k == 1 ? r >= 1 : (r >= k && r <= nz - k)) ? (T)fdc<FD>(k) : T(0)) *
+                               (sV[zc + k - 1][xc] - sV[zc - k][xc]);
                 }
             }
             if (!IZ && iz >= 0) dzv = pml_fwd_pre(dzv, mz3[rr], pz + 3 * a.psiz_n, a.profz, r, nz);
@@ -799,8 +799,10 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
 #pragma unroll
                 for (int k = 1; k <= NK; ++k) {
                     const T ck = (T)fdc<FD>(k);
-                    if (CX(c >= 1) && CZ(r >= k - 1 && r <= nz - 1 - k)) dzu += ck * (sU[zc + k][xc] - sU[zc - k + 1][xc]);
-                    if (CX(c >= k - 1 && c <= nx - 1 - k)) dxv += ck * (sV[zc][xc + k] - sV[zc][xc - k + 1]);
+                    // term masks as selects of the coefficient: no branches
+                    dzu += (CX(c >= 1) && CZ(r >= k - 1 && r <= nz - 1 - k) ? ck : T(0)) *
+                           (sU[zc + k][xc] - sU[zc - k + 1][xc]);
+                    dxv += (CX(c >= k - 1 && c <= nx - 1 - k) ? ck : T(0)) * (sV[zc][xc + k] - sV[zc][xc - k + 1]);
                 }
                 if (!IZ && iz >= 0) dzu = pml_fwd_pre(dzu, mz4[rr], pz + 4 * a.psiz_n, a.profz, r, nz);
                 if (!IX && ix >= 0) dxv = pml_fwd_pre(dxv, mx4[rr], px + 4 * a.psix_n, a.profx, c, nx);
