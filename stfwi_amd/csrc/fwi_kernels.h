@@ -161,13 +161,16 @@ __device__ __forceinline__ void stst(T *p, T v)
     __builtin_nontemporal_store(v, p);
 }
 
-// profile layout per axis: kappa[n], b[n], a[n], q[n] with q = 1/kappa + a
+// profile layout per axis: kappa[n], b[n], a[n], q[n], r[n] with
+// q = 1/kappa + a and r = 1/kappa.  The forward recursion multiplies by r
+// where the reference divides by kappa (one rounding apart; no divide
+// sequence in the edge tiles).
 template <typename T>
 __device__ __forceinline__ T pml_fwd(T D, T *psi, const T *prof, int j, int n)
 {
     const T pn = fma(prof[n + j], *psi, prof[2 * n + j] * D);   // psi' = b psi + a D
     *psi = pn;
-    return D / prof[j] + pn;                                   // D' = D / kappa + psi'
+    return D * prof[4 * n + j] + pn;                           // D' = D / kappa + psi'
 }
 
 template <typename T>
@@ -216,7 +219,7 @@ __device__ __forceinline__ T pml_fwd_pre(T D, T psi_old, T *psi, const T *prof, 
 {
     const T pn = fma(prof[n + j], psi_old, prof[2 * n + j] * D);
     *psi = pn;
-    return D / prof[j] + pn;
+    return D * prof[4 * n + j] + pn;
 }
 
 // ---------------------------------------------------------------------------
@@ -296,6 +299,7 @@ __global__ void k_profiles(int nz, int nx, int pad, int fs, int nzd, double dx, 
         p[n + j] = (T)b;
         p[2 * n + j] = (T)a;
         p[3 * n + j] = T(1) / (T)k + (T)a;
+        p[4 * n + j] = T(1) / (T)k;
     }
 }
 

@@ -218,8 +218,8 @@ extern "C" int stfwi_plan_create(const stfwi_config *cfg, stfwi_plan **out)
     const i64 S = c.n_shots_max;
     int rc = 0;
     rc |= dalloc(p, &p->mod, (4 * p->fstride + p->nx) * es);
-    rc |= dalloc(p, &p->profx, 4 * p->nx * es);
-    rc |= dalloc(p, &p->profz, 4 * p->nz * es);
+    rc |= dalloc(p, &p->profx, 5 * p->nx * es);
+    rc |= dalloc(p, &p->profz, 5 * p->nz * es);
     rc |= dalloc(p, (void **)&p->vpmax, sizeof(double));
     rc |= dalloc(p, &p->state, S * 5 * p->fstride * es);
     rc |= dalloc(p, &p->psix_f, S * 5 * p->psix_len * es);
