@@ -166,14 +166,6 @@ __device__ __forceinline__ void stst(T *p, T v)
 // where the reference divides by kappa (one rounding apart; no divide
 // sequence in the edge tiles).
 template <typename T>
-__device__ __forceinline__ T pml_fwd(T D, T *psi, const T *prof, int j, int n)
-{
-    const T pn = fma(prof[n + j], *psi, prof[2 * n + j] * D);   // psi' = b psi + a D
-    *psi = pn;
-    return D * prof[4 * n + j] + pn;                           // D' = D / kappa + psi'
-}
-
-template <typename T>
 __device__ __forceinline__ T pmlT_val(T G, T psi_old, const T *prof, int j, int n)
 {
     return fma(prof[3 * n + j], G, prof[2 * n + j] * psi_old); // (1/kappa + a) G + a psi
@@ -213,13 +205,27 @@ struct PmlIn {
     }
 };
 
-// pml_fwd with the old memory already loaded
+// the forward recursion's profile values (b, a, 1/kappa) at one column or
+// row, loaded once for all the derivatives that share it
 template <typename T>
-__device__ __forceinline__ T pml_fwd_pre(T D, T psi_old, T *psi, const T *prof, int j, int n)
+struct PmlFwd {
+    T b, a, r;
+};
+
+template <typename T>
+__device__ __forceinline__ PmlFwd<T> pml_fwd_coef(const T *prof, int j, int n)
 {
-    const T pn = fma(prof[n + j], psi_old, prof[2 * n + j] * D);
+    return PmlFwd<T>{prof[n + j], prof[2 * n + j], prof[4 * n + j]};
+}
+
+// the forward recursion psi' = b psi + a D, D' = D / kappa + psi' (the
+// division as a multiplication by 1/kappa)
+template <typename T>
+__device__ __forceinline__ T pml_fwd_apply(T D, T psi_old, T *psi, const PmlFwd<T> &k)
+{
+    const T pn = fma(k.b, psi_old, k.a * D);
     *psi = pn;
-    return D * prof[4 * n + j] + pn;
+    return D * k.r + pn;
 }
 
 // ---------------------------------------------------------------------------
@@ -572,6 +578,9 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
     __syncthreads();
     auto body = [&](auto tagx, auto tagz) {
         constexpr bool IX = decltype(tagx)::value, IZ = decltype(tagz)::value;
+        // x-profile values of this lane's column (clamped into the profile)
+        PmlFwd<T> kx{};
+        if (!IX) kx = pml_fwd_coef(a.profx, c < nx ? c : nx - 1, nx);
 #pragma unroll
         for (int rr = 0; rr < RPT; ++rr) {
             const int lz = ty + BLOCK_Y * rr, r = r0 + lz;
@@ -602,18 +611,19 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
                 const int ix = ixo[rr];
                 if (ix >= 0) {
                     T *px = a.psix + s * a.psix_stride + ix;
-                    D1 = pml_fwd_pre(D1, mx[rr][0], px + 0 * a.psix_n, a.profx, c, nx);
-                    D2 = pml_fwd_pre(D2, mx[rr][1], px + 1 * a.psix_n, a.profx, c, nx);
-                    D4 = pml_fwd_pre(D4, mx[rr][2], px + 2 * a.psix_n, a.profx, c, nx);
+                    D1 = pml_fwd_apply(D1, mx[rr][0], px + 0 * a.psix_n, kx);
+                    D2 = pml_fwd_apply(D2, mx[rr][1], px + 1 * a.psix_n, kx);
+                    D4 = pml_fwd_apply(D4, mx[rr][2], px + 2 * a.psix_n, kx);
                 }
             }
             if (!IZ) {
                 const int iz = izo[rr];
                 if (iz >= 0) {
                     T *pz = a.psiz + s * a.psiz_stride + iz;
-                    D3 = pml_fwd_pre(D3, mz[rr][0], pz + 0 * a.psiz_n, a.profz, r, nz);
-                    D5 = pml_fwd_pre(D5, mz[rr][1], pz + 1 * a.psiz_n, a.profz, r, nz);
-                    D6 = pml_fwd_pre(D6, mz[rr][2], pz + 2 * a.psiz_n, a.profz, r, nz);
+                    const PmlFwd<T> kz = pml_fwd_coef(a.profz, r, nz);
+                    D3 = pml_fwd_apply(D3, mz[rr][0], pz + 0 * a.psiz_n, kz);
+                    D5 = pml_fwd_apply(D5, mz[rr][1], pz + 1 * a.psiz_n, kz);
+                    D6 = pml_fwd_apply(D6, mz[rr][2], pz + 2 * a.psiz_n, kz);
                 }
             }
             const T b = b0[rr];
@@ -755,6 +765,9 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
     __syncthreads();
     auto body = [&](auto tagx, auto tagz) {
         constexpr bool IX = decltype(tagx)::value, IZ = decltype(tagz)::value;
+        // x-profile values of this lane's column (clamped into the profile)
+        PmlFwd<T> kx{};
+        if (!IX) kx = pml_fwd_coef(a.profx, c < nx ? c : nx - 1, nx);
 #pragma unroll
         for (int rr = 0; rr < RPT; ++rr) {
             const int lz = ty + BLOCK_Y * rr, r = r0 + lz;
@@ -772,7 +785,10 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
             const int ix = IX ? -1 : ixo[rr], iz = IZ ? -1 : izo[rr];
             T *px = a.psix + s * a.psix_stride + ix;
             T *pz = a.psiz + s * a.psiz_stride + iz;
-            if (!IX && ix >= 0) dxu = pml_fwd_pre(dxu, mx3[rr], px + 3 * a.psix_n, a.profx, c, nx);
+            if (!IX && ix >= 0) dxu = pml_fwd_apply(dxu, mx3[rr], px + 3 * a.psix_n, kx);
+            // z-profile values of this row, for D-z V and D+z U
+            PmlFwd<T> kz{};
+            if (!IZ && iz >= 0) kz = pml_fwd_coef(a.profz, r, nz);
             T exn = CX(c >= 1) ? ex0[rr] + g.dtdx * dxu : ex0[rr];
             // ezz: D-z V on columns [0, nx-2]; free-surface row from the C-PML'd D-x U
             T dzv = 0;
@@ -786,7 +802,7 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
                                (sV[zc + k - 1][xc] - sV[zc - k][xc]);
                 }
             }
-            if (!IZ && iz >= 0) dzv = pml_fwd_pre(dzv, mz3[rr], pz + 3 * a.psiz_n, a.profz, r, nz);
+            if (!IZ && iz >= 0) dzv = pml_fwd_apply(dzv, mz3[rr], pz + 3 * a.psiz_n, kz);
             T ezn = ez0[rr] + g.dtdz * dzv;
             if (sh.type == 1 && r == sh.sz && c == sh.sx) {
                 const T lm = (a.L2M[g.origin + q] - T(2) * a.M[g.origin + q]) + a.M[g.origin + q];
@@ -808,8 +824,8 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
                            (sU[zc + k][xc] - sU[zc - k + 1][xc]);
                     dxv += (CX(c >= k - 1 && c <= nx - 1 - k) ? ck : T(0)) * (sV[zc][xc + k] - sV[zc][xc - k + 1]);
                 }
-                if (!IZ && iz >= 0) dzu = pml_fwd_pre(dzu, mz4[rr], pz + 4 * a.psiz_n, a.profz, r, nz);
-                if (!IX && ix >= 0) dxv = pml_fwd_pre(dxv, mx4[rr], px + 4 * a.psix_n, a.profx, c, nx);
+                if (!IZ && iz >= 0) dzu = pml_fwd_apply(dzu, mz4[rr], pz + 4 * a.psiz_n, kz);
+                if (!IX && ix >= 0) dxv = pml_fwd_apply(dxv, mx4[rr], px + 4 * a.psix_n, kx);
                 esn = esn + T(0.5) * fma(g.dtdz, dzu, g.dtdx * dxv);
                 stgg(es, q, esn);
             }
