@@ -203,6 +203,29 @@ struct PmlIn {
         a = ld(prof + 2 * n, jc);
         b = ld(prof + n, jc);
     }
+    // the two memories only; the profile values copied from `k`, loaded
+    // once for all the cells of one column
+    template <bool BUF>
+    __device__ __forceinline__ void load_mem(const T *mem0, const T *mem1, int i, const PmlIn &k)
+    {
+        const int ic = i >= 0 ? i : 0;
+        auto ld = [](const T *b_, int kk) { return BUF ? ldg(b_, kk) : ldgg(b_, kk); };
+        m0 = ld(mem0, ic);
+        m1 = ld(mem1, ic);
+        q = k.q;
+        a = k.a;
+        b = k.b;
+    }
+    // the profile values of column (or row) j only
+    template <bool BUF>
+    __device__ __forceinline__ void load_coef(const T *prof, int j, int n)
+    {
+        const int jc = j < 0 ? 0 : (j >= n ? n - 1 : j);
+        auto ld = [](const T *b_, int kk) { return BUF ? ldg(b_, kk) : ldgg(b_, kk); };
+        q = ld(prof + 3 * n, jc);
+        a = ld(prof + 2 * n, jc);
+        b = ld(prof + n, jc);
+    }
 };
 
 // the forward recursion's profile values (b, a, 1/kappa) at one column or
@@ -1123,6 +1146,8 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
         }
     } else {
         // z-slab: Dz_U_a, Dz_V_a everywhere, Dx_V_a, Dx_U_a on the tile rows
+        PmlIn<T> kxc{};   // x-profile values of this lane's column
+        if (!IX) kxc.template load_coef<true>(a.profx, c, nx);
 #pragma unroll
         for (int k = 0; k < KZ; ++k) {
             const int r = r0 - 4 + ty + BLOCK_Y * k;
@@ -1130,7 +1155,7 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
             ixs[k] = (!IX && zok[k]) ? xstrip(g, r, c) : -1;
             if (!IZ) pz[k].template load<true>(S.pzo, S.pzo + S.pzn, izs[k], a.profz, r, nz);
             if (!IX && k >= 1 && k < TZ / BLOCK_Y + 1)
-                px[k].template load<true>(S.pxo, S.pxo + S.pxn, ixs[k], a.profx, c, nx);
+                px[k].template load_mem<true>(S.pxo, S.pxo + S.pxn, ixs[k], kxc);
         }
         PmlIn<T> hp[KX];
         int ihs[KX];
@@ -1526,6 +1551,8 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
         }
     } else {
         // memories: x[2] = Dx_M_es, x[3] = Dx_(B U); z[2] = Dz_(B V), z[3] = Dz_M_es
+        PmlIn<T> kxc{};   // x-profile values of this lane's column
+        if (!IX) kxc.template load_coef<false>(a.profx, c, nx);
 #pragma unroll
         for (int k = 0; k < KZ; ++k) {
             const int r = r0 - 4 + ty + BLOCK_Y * k;
@@ -1533,7 +1560,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
             ixs[k] = (!IX && zok[k]) ? xstrip(g, r, c) : -1;
             if (!IZ) pz[k].template load<false>(S.pzo + 2 * S.pzn, S.pzo + 3 * S.pzn, izs[k], a.profz, r, nz);
             if (!IX && k >= 1 && k < TZ / BLOCK_Y + 1)
-                px[k].template load<false>(S.pxo + 2 * S.pxn, S.pxo + 3 * S.pxn, ixs[k], a.profx, c, nx);
+                px[k].template load_mem<false>(S.pxo + 2 * S.pxn, S.pxo + 3 * S.pxn, ixs[k], kxc);
         }
         PmlIn<T> hp[KX];
         int ihs[KX];
