@@ -403,3 +403,28 @@ def test_gpu_lean_kernels_agree_bitwise(dtype):
         outs.append((rec.clone(), sa, g1))
     for x, y in zip(outs[0], outs[1]):
         assert torch.equal(x, y)
+
+
+def test_gpu_grouped_autotune_keeps_results():
+    """The autotuner times its candidates with the plan's stream groups; the
+    tile heights it picks give the same records and gradients bit for bit
+    as fixed heights, and the plan runs normally after tuning."""
+    m, rx, rz, wav = _big_case()
+    rng = np.random.default_rng(9)
+    obs = torch.as_tensor(rng.standard_normal((3, m["nt"], rx.size)) * 1e-9, dtype=torch.float32)
+    p = _plan(m, 8, True, torch.float32, nshots=3, nrec=rx.size, imaging="exact", grad="vs", hist=True)
+    p.set_model(m["Vs"], m["Vp"], m["rho"])
+    p.set_shots([40, 90, 140], [3, 5, 2], "moment", [m["nt"], m["nt"] - 11, m["nt"]], rx, rz, wav)
+    p.set_groups(3)
+    tiles, table = p.autotune(2)
+    assert all(t in (4, 8, 16, 32) for t in tiles)
+    assert set(table) == {"fwd_vel", "fwd_strain", "adj_strain", "adj_vel"}
+    outs = []
+    for tz in (tiles, [8] * 4):
+        p.set_tiles(tz)
+        rec = p.forward()["rec"]
+        _, radj = p.residual(rec, obs, 2, None, 0, 1e15, 3)
+        p.adjoint(radj, want_src=False)
+        outs.append((rec.clone(), p.finalize(m["Vs"], m["Vp"], m["rho"]).clone()))
+    assert torch.equal(outs[0][0], outs[1][0])
+    assert torch.equal(outs[0][1], outs[1][1])
