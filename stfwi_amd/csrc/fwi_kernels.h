@@ -155,10 +155,12 @@ __device__ __forceinline__ void stgg(T *base, int idx, T v)
 // stst: store of a write-once stream (the forward history, snapshots) with
 // the non-temporal hint, so that these streams do not displace the resident
 // wavefield state from the caches; they are read back many steps later.
+// base is block-uniform and idx a 32-bit offset from it, as for stgg.
 template <typename T>
-__device__ __forceinline__ void stst(T *p, T v)
+__device__ __forceinline__ void stst(T *base, int idx, T v)
 {
-    __builtin_nontemporal_store(v, p);
+    const uint32_t q = opaque_off((uint32_t)idx * (uint32_t)sizeof(T));
+    __builtin_nontemporal_store(v, reinterpret_cast<T *>(reinterpret_cast<char *>(base) + q));
 }
 
 // profile layout per axis: kappa[n], b[n], a[n], q[n], r[n] with
@@ -662,9 +664,10 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
             if (a.hist2_mode == 1) {
                 const int rp = r - g.z0, cp = c - g.pad;
                 if (CZ(rp >= 0 && rp < g.nzd) && CX(cp >= 0 && cp < g.nxd)) {
-                    T *h = a.hist2 + ((i64)s * a.hist_nt + (it - a.hist_t0)) * 2 * g.np + (i64)rp * g.nxd + cp;
-                    stst<T>(h, fma(g.dtdz, D3, fma(g.dtdx, D2, g.dtdx * D1)));
-                    stst<T>(h + g.np, fma(g.dtdz, D6, fma(g.dtdz, D5, g.dtdx * D4)) + sv);
+                    T *h = a.hist2 + ((i64)s * a.hist_nt + (it - a.hist_t0)) * 2 * g.np;
+                    const int o = rp * g.nxd + cp;
+                    stst<T>(h, o, fma(g.dtdz, D3, fma(g.dtdx, D2, g.dtdx * D1)));
+                    stst<T>(h, o + (int)g.np, fma(g.dtdz, D6, fma(g.dtdz, D5, g.dtdx * D4)) + sv);
                 }
             }
         }
@@ -857,13 +860,13 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
             if (a.hist && CZ(rp >= 0 && rp < g.nzd) && CX(cp >= 0 && cp < g.nxd)) {
                 T *hb = a.hist + ((i64)s * a.hist_nt + (it - a.hist_t0)) * 3 * g.np;
                 const int o = rp * g.nxd + cp, npi = (int)g.np;
-                stst(hb + o, exn);
-                stst(hb + o + npi, ezn);
-                stst(hb + o + 2 * npi, esn);
+                stst(hb, o, exn);
+                stst(hb, o + npi, ezn);
+                stst(hb, o + 2 * npi, esn);
                 if (a.hist2_mode == 2) {
-                    T *h = a.hist2 + ((i64)s * a.hist_nt + (it - a.hist_t0)) * 2 * g.np + (i64)rp * g.nxd + cp;
-                    stst(h, sU[zc][xc]);
-                    stst(h + g.np, sV[zc][xc]);
+                    T *h = a.hist2 + ((i64)s * a.hist_nt + (it - a.hist_t0)) * 2 * g.np;
+                    stst(h, o, sU[zc][xc]);
+                    stst(h, o + npi, sV[zc][xc]);
                 }
             }
             const i64 pc = (i64)r * nx + c;
@@ -876,7 +879,7 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
                 int k = 0;
                 for (int f = 0; f < 5; ++f)
                     if (a.snap_mask & (1 << f)) {
-                        stst(a.snap + (((i64)s * a.nsnap + k) * a.nt_max + it) * nz * nx + pc, vals[f]);
+                        stst(a.snap + (((i64)s * a.nsnap + k) * a.nt_max + it) * nz * nx, (int)pc, vals[f]);
                         ++k;
                     }
             }
@@ -1714,7 +1717,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
                     int k = 0;
                     for (int f = 0; f < 5; ++f)
                         if (a.snap_mask & (1 << f)) {
-                            stst(a.snap + (((i64)s * a.nsnap + k) * a.nt_max + it) * nz * nx + (i64)r * nx + c, vals[f]);
+                            stst(a.snap + (((i64)s * a.nsnap + k) * a.nt_max + it) * nz * nx, r * nx + c, vals[f]);
                             ++k;
                         }
                 }
