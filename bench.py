@@ -176,8 +176,9 @@ def main():
                          "256 MiB device copy; for hardware-counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU design "
-              "(shot-parallel with one all-reduce per gradient) is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); more GPUs would split the "
+              "shots, with one all-reduce of the gradient and misfit per evaluation, and that path "
+              "is not built", file=sys.stderr)
         sys.exit(2)
     import torch
     from stfwi_amd import Plan
