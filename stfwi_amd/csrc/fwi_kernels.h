@@ -1456,7 +1456,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
         hu[k] = ldgg(Ua, p);
         hv[k] = ldgg(Va, p);
     }
-    // owned values and the history frame, loaded ahead of the barrier
+    // owned values, loaded ahead of the barrier
     T l2m0[RPT], mu0[RPT], exa0v[RPT], eza0v[RPT], esa0v[RPT], h0[RPT], h1[RPT], h2v[RPT], ac0[RPT], ac1[RPT];
     const i64 hs = (i64)s * a.nt_max;            // wavelet / dxu0 rows
     const i64 hh = (i64)s * a.hist_nt - a.hist_t0; // history frame base
@@ -1470,24 +1470,6 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
         exa0v[rr] = ldgg((const T *)exa, q);
         eza0v[rr] = ldgg((const T *)eza, q);
         esa0v[rr] = ldgg((const T *)esa, q);
-        // history frame: read wherever the imaging below uses it (physical
-        // cells; frame -1 is never used); other lanes read entry 0
-        h0[rr] = h1[rr] = h2v[rr] = ac0[rr] = ac1[rr] = T(0);
-        if (a.imaging) {
-            const int rp = r - g.z0, cp = c - g.pad;
-            const bool pok = rp >= 0 && rp < g.nzd && cp >= 0 && cp < g.nxd;
-            const bool hok = hframe >= 0 && pok;
-            const T *hfr = a.hist + (hh + (hframe >= a.hist_t0 ? hframe : a.hist_t0)) * 3 * g.np;
-            const int o = hok ? rp * g.nxd + cp : 0, npi = (int)g.np;
-            h0[rr] = ldgg(hfr, o);
-            h1[rr] = ldgg(hfr, o + npi);
-            h2v[rr] = ldgg(hfr, o + 2 * npi);
-            // first two gradient accumulators of the cell
-            const T *accb = a.gacc + (i64)s * a.nacc * g.np;
-            const int oa = pok ? rp * g.nxd + cp : 0;
-            ac0[rr] = ldgg(accb, oa);
-            if (a.gparams >= 2) ac1[rr] = ldgg(accb, oa + npi);
-        }
     }
     // receiver adjoints of channels exx, ezz of step it-1, added to the
     // strains this step stores (receiver tiles only)
@@ -1628,6 +1610,30 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
     };
     STFWI_DISPATCH(inx, inz, stage);
     __syncthreads();
+    // history frame and accumulators, issued after the barrier: their
+    // registers are not live across the staging
+#pragma unroll
+    for (int rr = 0; rr < RPT; ++rr) {
+        const int r = r0 + ty + BLOCK_Y * rr;
+        // history frame: read wherever the imaging below uses it (physical
+        // cells; frame -1 is never used); other lanes read entry 0
+        h0[rr] = h1[rr] = h2v[rr] = ac0[rr] = ac1[rr] = T(0);
+        if (a.imaging) {
+            const int rp = r - g.z0, cp = c - g.pad;
+            const bool pok = rp >= 0 && rp < g.nzd && cp >= 0 && cp < g.nxd;
+            const bool hok = hframe >= 0 && pok;
+            const T *hfr = a.hist + (hh + (hframe >= a.hist_t0 ? hframe : a.hist_t0)) * 3 * g.np;
+            const int o = hok ? rp * g.nxd + cp : 0, npi = (int)g.np;
+            h0[rr] = ldgg(hfr, o);
+            h1[rr] = ldgg(hfr, o + npi);
+            h2v[rr] = ldgg(hfr, o + 2 * npi);
+            // first two gradient accumulators of the cell
+            const T *accb = a.gacc + (i64)s * a.nacc * g.np;
+            const int oa = pok ? rp * g.nxd + cp : 0;
+            ac0[rr] = ldgg(accb, oa);
+            if (a.gparams >= 2) ac1[rr] = ldgg(accb, oa + npi);
+        }
+    }
     auto body = [&](auto tagx, auto tagz) {
         constexpr bool IX = decltype(tagx)::value, IZ = decltype(tagz)::value;
         // the column (x) term masks as per-lane weights, set once per tile
