@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define STFWI_ABI_VERSION 1
+#define STFWI_ABI_VERSION 2
 
 typedef enum {
     STFWI_OK = 0,
@@ -66,6 +66,12 @@ typedef struct stfwi_config {
                            * reference's jax.checkpoint blocks,
                            * fwi/forward.py:471-501).  Results do not depend
                            * on K. */
+    long long hist_frames;  /* history capacity in frames summed over the
+                           * shots of one stfwi_set_shots (each shot keeps
+                           * its own nt frames, so uneven records do not pay
+                           * for the longest); 0 = n_shots_max * nt_max.
+                           * Ignored with ckpt_every > 0 (K + 1 frames a
+                           * shot). */
 } stfwi_config;
 
 /* One shot.  Indices are physical-domain (the reference's src_x, src_z,
@@ -90,6 +96,21 @@ int stfwi_plan_create(const stfwi_config *cfg, stfwi_plan **out);
 int stfwi_plan_destroy(stfwi_plan *plan);
 /* bytes of device memory the plan holds (host call) */
 long long stfwi_plan_bytes(const stfwi_plan *plan);
+
+/* Bytes of device memory stfwi_plan_create would allocate for cfg, without
+ * allocating (host call; no reference counterpart). */
+int stfwi_plan_footprint(const stfwi_config *cfg, long long *bytes);
+
+/* Checkpoint planner (no reference counterpart; the reference fixes its
+ * jax.checkpoint block size by hand, fwi/forward.py:471-501,
+ * Main_strain_fwi.py:42-46): *ckpt_every receives 0 when the plan of cfg
+ * with the whole history fits in budget_bytes, otherwise the segment
+ * length K whose plan has the smallest footprint (about sqrt(nt_max) state
+ * checkpoints plus one segment of history).  STFWI_ERR_OOM when no K fits;
+ * STFWI_ERR_UNSUPPORTED when checkpointing would be needed but the imaging
+ * mode does not allow it (Zhou imaging of rho).  cfg->ckpt_every is
+ * ignored. */
+int stfwi_plan_choose_ckpt(const stfwi_config *cfg, long long budget_bytes, int *ckpt_every);
 
 /* Model: vs, vp, rho (nz_dom x nx_dom, plan dtype).  Builds the padded
  * moduli, buoyancy and the 1-D C-PML profiles on the device
@@ -208,9 +229,13 @@ int stfwi_zhou_imaging(const void *ex, const void *ez, const void *es, const voi
 /* Diagnostics (no reference counterpart): time `reps` back-to-back launches
  * of one time-step kernel over all shots of the plan (which: 0 forward
  * velocity, 1 forward strain, 2 adjoint strain, 3 adjoint velocity with
- * imaging) with HIP events on `stream`; *ms receives the total.  Host call,
- * synchronous.  It overwrites the plan's wavefield state. */
-int stfwi_time_kernel(stfwi_plan *plan, int which, int reps, void *stream, double *ms);
+ * imaging) with HIP events on `stream`; *ms receives the total.  With
+ * grouped != 0 each repetition launches every shot group of the plan on its
+ * own stream, as the time loops run them; otherwise one launch covers all
+ * shots.  Host call, synchronous.  It overwrites the plan's wavefield state,
+ * so a forward or adjoint that ran before it has to run again before the
+ * next adjoint or finalize. */
+int stfwi_time_kernel(stfwi_plan *plan, int which, int reps, int grouped, void *stream, double *ms);
 
 /* Tile-shape autotuner (no reference counterpart): for each of the four
  * time-step kernels time `reps` launches at every compiled tile height
@@ -226,13 +251,10 @@ int stfwi_autotune(stfwi_plan *plan, int reps, void *stream, int *tz_out, double
  * forward strain, adjoint strain, adjoint velocity). */
 int stfwi_set_tiles(stfwi_plan *plan, const int *tz);
 
-/* Graph replay of the time loops (default off; no reference counterpart):
- * stfwi_forward without illumination or snapshots and stfwi_adjoint without
- * snapshots capture their whole time loop into a hipGraph the first time a
- * launch configuration (shots, record length, tiles, groups, channels) is
- * seen and replay it afterwards, with receivers, receiver adjoints and the
- * source adjoint staged through plan-owned buffers.  Results are identical
- * to the eager launches; enable = 0 (the default) launches eagerly. */
+/* Graph replay of the time loops (no reference counterpart): refused.
+ * enable != 0 returns STFWI_ERR_UNSUPPORTED, because a host segmentation
+ * fault with replay on has no cause shown yet; enable = 0 keeps the eager
+ * launches, the only path. */
 int stfwi_set_graphs(stfwi_plan *plan, int enable);
 
 /* Adjoint wavefield energy (no reference counterpart; the Wr that eprecond1,

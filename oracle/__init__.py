@@ -12,7 +12,9 @@ Layout
                             (fwi/imaging.py:83-169) and the exact discrete
                             gradient, built as ``liborc.so`` for float32,
                             float64 and complex128.
-``core.py``                 ctypes wrappers around ``liborc.so``.
+``core.py``                 ctypes wrappers around ``liborc.so``, and
+                            ``zhou_cubes``: the Zhou imaging of
+                            fwi/imaging.py:111-152 from history cubes in numpy.
 ``signal.py``               Ricker, cosine end-taper, DF-I biquad cascade and
                             its transpose, residual prep (fwi/filters.py,
                             fwi/gradient.py:139-156).
@@ -24,8 +26,10 @@ Layout
 
 Parity status of each function is recorded in the module docstrings; every
 function listed above is pinned by a test under tests/ (fixtures, closed
-forms, dot-product, complex-step), none is "parity unpinned".
+forms, dot-product, complex-step).  The Zhou density term is pinned by
+closed forms of ``zhou_cubes`` on constructed cubes and by the fused C
+restatement agreeing with ``zhou_cubes`` on its own adjoint cubes.
 """
 from .core import (Geom, build, forward, adjoint, setup, padded_shape,  # noqa: F401
-                   exact_chain, zhou_chain)
+                   exact_chain, zhou_chain, zhou_cubes)
 from . import signal, precond, gradient  # noqa: F401

@@ -215,3 +215,35 @@ def zhou_chain(g: Geom, gacc, vs, vp, rho):
     g_vp = 2.0 * rho * vp * g_lam
     g_rho = (vp ** 2 - 2.0 * vs ** 2) * g_lam + vs ** 2 * g_mu + g_rl
     return g_vs, g_vp, g_rho
+
+
+def zhou_cubes(ex, ez, es, ua, va, b_full, dx, dz, dt, vx=None, vz=None):
+    """Zhou imaging from history cubes on the padded grid, before cropping
+    (fwi/imaging.py:111-152): returns (g_lam, g_mu, g_rho_lame), each
+    (nz_t, nx_t), with the reference's sign.
+
+    Spatial derivatives are centred differences with periodic wrap
+    (jnp.roll, fwi/imaging.py:47-53) of the buoyancy-weighted adjoint
+    velocities; the density term pairs the forward velocity of step t with
+    the centred time difference of the adjoint velocity over t-1 and t+1,
+    for t = 1 .. nt-2 (fwi/imaging.py:145-152)."""
+    ex, ez, es, ua, va = (np.asarray(a, np.float64) for a in (ex, ez, es, ua, va))
+    b = np.asarray(b_full, np.float64)
+    bu, bv = b * ua, b * va
+
+    def ddx(f):
+        return (np.roll(f, -1, axis=2) - np.roll(f, 1, axis=2)) / (2.0 * dx)
+
+    def ddz(f):
+        return (np.roll(f, -1, axis=1) - np.roll(f, 1, axis=1)) / (2.0 * dz)
+
+    dxbu, dzbv, dzbu, dxbv = ddx(bu), ddz(bv), ddz(bu), ddx(bv)
+    g_lam = -dt * np.sum((ex + ez) * (dxbu + dzbv), axis=0)
+    g_mu = -dt * np.sum(2.0 * ex * dxbu + 2.0 * ez * dzbv + es * (dzbu + dxbv), axis=0)
+    g_rl = np.zeros_like(g_lam)
+    if vx is not None and vz is not None:
+        vx, vz = np.asarray(vx, np.float64), np.asarray(vz, np.float64)
+        dua = (ua[2:] - ua[:-2]) / (2.0 * dt)
+        dva = (va[2:] - va[:-2]) / (2.0 * dt)
+        g_rl = -dt * np.sum(vx[1:-1] * dua + vz[1:-1] * dva, axis=0)
+    return g_lam, g_mu, g_rl

@@ -11,6 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libstfwi.so")
 
+ABI_VERSION = 2
 F32, F64 = 0, 1
 SRC_FORCE, SRC_MOMENT = 0, 1
 IMAGING = {"none": 0, "zhou": 1, "exact": 2}
@@ -26,12 +27,14 @@ SIGNATURES = {
     "stfwi_plan_create": (C.c_int, [_VP, C.POINTER(_VP)]),
     "stfwi_plan_destroy": (C.c_int, [_VP]),
     "stfwi_plan_bytes": (C.c_longlong, [_VP]),
+    "stfwi_plan_footprint": (C.c_int, [_VP, C.POINTER(C.c_longlong)]),
+    "stfwi_plan_choose_ckpt": (C.c_int, [_VP, C.c_longlong, C.POINTER(C.c_int)]),
     "stfwi_set_model": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP]),
     "stfwi_set_shots": (C.c_int, [_VP, _VP, C.c_int, _VP, _VP, _VP, _VP]),
     "stfwi_forward": (C.c_int, [_VP, _VP, C.c_int, _VP, C.c_int, _VP, C.c_int, _VP]),
     "stfwi_adjoint": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int, _VP]),
     "stfwi_gradient_finalize": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int, _VP, _VP]),
-    "stfwi_time_kernel": (C.c_int, [_VP, C.c_int, C.c_int, _VP, C.POINTER(C.c_double)]),
+    "stfwi_time_kernel": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, _VP, C.POINTER(C.c_double)]),
     "stfwi_autotune": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP]),
     "stfwi_set_tiles": (C.c_int, [_VP, _VP]),
     "stfwi_set_groups": (C.c_int, [_VP, C.c_int]),
@@ -80,7 +83,7 @@ class Config(C.Structure):
                 ("fd_order", C.c_int), ("dx", C.c_double), ("dz", C.c_double), ("dt", C.c_double),
                 ("fc", C.c_double), ("dtype", C.c_int), ("n_shots_max", C.c_int), ("nt_max", C.c_int),
                 ("nrec_max", C.c_int), ("imaging", C.c_int), ("grad_params", C.c_int),
-                ("store_history", C.c_int), ("ckpt_every", C.c_int)]
+                ("store_history", C.c_int), ("ckpt_every", C.c_int), ("hist_frames", C.c_longlong)]
 
 
 class Shot(C.Structure):
@@ -107,7 +110,7 @@ def lib():
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
-        if L.stfwi_abi_version() != 1:
+        if L.stfwi_abi_version() != ABI_VERSION:
             raise StfwiError("libstfwi.so ABI version mismatch")
         _lib = L
     return _lib

@@ -46,6 +46,7 @@ struct ShotDev {
     int sx, sz;        // padded-grid source cell
     int type;          // 0 force, 1 moment
     int nt;            // record length
+    int hoff;          // first history frame of the shot (frames packed by shot)
     int rec_off, nrec; // receivers [rec_off, rec_off + nrec) of rec_cell
     int uc_off, nuc;   // merged receiver cells [uc_off, uc_off + nuc)
 };
@@ -386,9 +387,9 @@ struct FwdArgs {
     T *rec;                          // [shot][4][nt_max][nrec_max]
     int nrec_max, rec_mask;
     const i64 *rec_cell;             // cell offset r*ld + c of every receiver
-    T *hist;                         // [shot][hist_nt][3][np] strains
-    T *hist2;                        // [shot][hist_nt][2][np]: 1 = dU/dB,dV/dB  2 = U,V
-    int hist_nt, hist_t0;            // frames per shot; step of frame 0
+    T *hist;                         // [frame][3][np] strains, shot s from frame hoff
+    T *hist2;                        // [frame][2][np]: 1 = dU/dB,dV/dB  2 = U,V
+    int hist_nt, hist_t0;            // frames per shot with checkpoints; step of frame 0
     int hist2_mode;
     T *dxu0;                         // [shot][nt_max][nx]
     T *illum;                        // [shot][nz*nx]
@@ -664,7 +665,7 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
             if (a.hist2_mode == 1) {
                 const int rp = r - g.z0, cp = c - g.pad;
                 if (CZ(rp >= 0 && rp < g.nzd) && CX(cp >= 0 && cp < g.nxd)) {
-                    T *h = a.hist2 + ((i64)s * a.hist_nt + (it - a.hist_t0)) * 2 * g.np;
+                    T *h = a.hist2 + ((i64)sh.hoff + (it - a.hist_t0)) * 2 * g.np;
                     const int o = rp * g.nxd + cp;
                     stst<T>(h, o, fma(g.dtdz, D3, fma(g.dtdx, D2, g.dtdx * D1)));
                     stst<T>(h, o + (int)g.np, fma(g.dtdz, D6, fma(g.dtdz, D5, g.dtdx * D4)) + sv);
@@ -858,13 +859,13 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
             if (!IZ && a.dxu0 && r == 0) a.dxu0[((i64)s * a.nt_max + it) * nx + c] = dxu;
             const int rp = r - g.z0, cp = c - g.pad;
             if (a.hist && CZ(rp >= 0 && rp < g.nzd) && CX(cp >= 0 && cp < g.nxd)) {
-                T *hb = a.hist + ((i64)s * a.hist_nt + (it - a.hist_t0)) * 3 * g.np;
+                T *hb = a.hist + ((i64)sh.hoff + (it - a.hist_t0)) * 3 * g.np;
                 const int o = rp * g.nxd + cp, npi = (int)g.np;
                 stst(hb, o, exn);
                 stst(hb, o + npi, ezn);
                 stst(hb, o + 2 * npi, esn);
                 if (a.hist2_mode == 2) {
-                    T *h = a.hist2 + ((i64)s * a.hist_nt + (it - a.hist_t0)) * 2 * g.np;
+                    T *h = a.hist2 + ((i64)sh.hoff + (it - a.hist_t0)) * 2 * g.np;
                     stst(h, o, sU[zc][xc]);
                     stst(h, o + npi, sV[zc][xc]);
                 }
@@ -916,8 +917,8 @@ struct AdjArgs {
     int nuc_max;
     T *src_adj;                        // [shot][nt_max]
     int imaging, gparams, nacc;
-    const T *hist, *hist2;             // [shot][hist_nt][3 | 2][np]
-    int hist_nt, hist_t0;              // frames per shot; step of frame 0
+    const T *hist, *hist2;             // [frame][3 | 2][np], shot s from frame hoff
+    int hist_nt, hist_t0;              // frames per shot with checkpoints; step of frame 0
     const T *dxu0;
     T *gacc;                           // [shot][nacc][np]
     T *snap;
@@ -1459,7 +1460,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
     // owned values, loaded ahead of the barrier
     T l2m0[RPT], mu0[RPT], exa0v[RPT], eza0v[RPT], esa0v[RPT], h0[RPT], h1[RPT], h2v[RPT], ac0[RPT], ac1[RPT];
     const i64 hs = (i64)s * a.nt_max;            // wavelet / dxu0 rows
-    const i64 hh = (i64)s * a.hist_nt - a.hist_t0; // history frame base
+    const i64 hh = (i64)sh.hoff - a.hist_t0;      // history frame base
     const int hframe = a.imaging == 2 ? it - 1 : it;
 #pragma unroll
     for (int rr = 0; rr < RPT; ++rr) {
@@ -1821,15 +1822,17 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
 // buffers (the state restored from a checkpoint is the carry-in of the
 // segment's first step)
 template <typename T>
-__global__ void k_hist_from_state(Geo<T> g, const T *state, i64 sstride, T *hist, int hist_nt, int frame, int shot0)
+__global__ void k_hist_from_state(Geo<T> g, const T *state, i64 sstride, T *hist, const ShotDev *shots, int frame,
+                                  int shot0)
 {
     const int s = shot0 + blockIdx.y;
+    const int hoff = shots[s].hoff;
     const i64 np = g.np;
     for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < np; i += (i64)gridDim.x * blockDim.x) {
         const int rp = (int)(i / g.nxd), cp = (int)(i - (i64)rp * g.nxd);
         const i64 cell = g.origin + (i64)(rp + g.z0) * g.ld + cp + g.pad;
         const T *st = state + s * sstride;
-        T *h = hist + ((i64)s * hist_nt + frame) * 3 * np + i;
+        T *h = hist + ((i64)hoff + frame) * 3 * np + i;
         h[0] = st[2 * g.fstride + cell];
         h[np] = st[3 * g.fstride + cell];
         h[2 * np] = st[4 * g.fstride + cell];

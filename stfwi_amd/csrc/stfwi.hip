@@ -65,6 +65,9 @@ struct stfwi_plan {
     // recomputes each earlier segment from its checkpoint.  `state_a` is the
     // adjoint's own state buffer in that mode.
     int ck = 0, hist_nt = 0;
+    i64 hist_cap = 0;    // history frames over all shots of one set_shots
+    std::vector<int> shot_hoff;   // first history frame of each shot
+    bool dry = false;    // size the allocations only (stfwi_plan_footprint)
     void *ckpt = nullptr, *state_a = nullptr;
     i64 ck_state = 0, ck_psix = 0, ck_psiz = 0;   // elements per slot region
     int tz[4] = {16, 16, 16, 16};   // tile height per time-step kernel (autotuned)
@@ -109,6 +112,11 @@ struct stfwi_plan {
 static int dalloc(stfwi_plan *p, void **ptr, size_t bytes)
 {
     if (bytes == 0) bytes = 256;
+    if (p->dry) {
+        *ptr = nullptr;
+        p->bytes += (i64)bytes;
+        return 0;
+    }
     hipError_t e = hipMalloc(ptr, bytes);
     if (e != hipSuccess) {
         *ptr = nullptr;
@@ -178,11 +186,8 @@ extern "C" int stfwi_plan_destroy(stfwi_plan *plan)
     return 0;
 }
 
-extern "C" int stfwi_plan_create(const stfwi_config *cfg, stfwi_plan **out)
+static int validate_config(const stfwi_config &c)
 {
-    if (!cfg || !out) return fail(STFWI_ERR_INVALID, "null argument");
-    *out = nullptr;
-    const stfwi_config &c = *cfg;
     if (c.fd_order != 2 && c.fd_order != 4 && c.fd_order != 8)
         return fail(STFWI_ERR_UNSUPPORTED, "fd_order=%d not supported (use 2, 4, or 8)", c.fd_order);
     if (c.dtype != STFWI_F32 && c.dtype != STFWI_F64) return fail(STFWI_ERR_INVALID, "dtype=%d", c.dtype);
@@ -192,19 +197,23 @@ extern "C" int stfwi_plan_create(const stfwi_config *cfg, stfwi_plan **out)
         return fail(STFWI_ERR_INVALID, "imaging=%d grad_params=%d", c.imaging, c.grad_params);
     if (c.dx <= 0 || c.dz <= 0 || c.dt <= 0) return fail(STFWI_ERR_INVALID, "dx, dz, dt must be positive");
     if (c.ckpt_every < 0) return fail(STFWI_ERR_INVALID, "ckpt_every=%d", c.ckpt_every);
+    if (c.hist_frames < 0) return fail(STFWI_ERR_INVALID, "hist_frames=%lld", c.hist_frames);
     if (c.ckpt_every > 0 && c.imaging == STFWI_IMAGING_ZHOU && c.grad_params == STFWI_GRAD_ALL)
         return fail(STFWI_ERR_UNSUPPORTED, "checkpointing does not support Zhou imaging of rho (it pairs steps "
                                            "across segment ends)");
-    stfwi_plan *p = new stfwi_plan();
-    p->cfg = c;
+    const int nz = c.nz_dom + (c.free_surface ? c.pad : 2 * c.pad), nx = c.nx_dom + 2 * c.pad;
+    if (nz < 10 || nx < 10) return fail(STFWI_ERR_INVALID, "padded grid %dx%d too small", nz, nx);
+    return 0;
+}
+
+// geometry and every device allocation of a plan (sizes only when p->dry)
+static int plan_alloc(stfwi_plan *p)
+{
+    const stfwi_config &c = p->cfg;
     p->esz = c.dtype == STFWI_F64 ? 8 : 4;
     for (int k = 0; k < 4; ++k) p->tz[k] = c.dtype == STFWI_F64 ? 8 : 16;
     p->nz = c.nz_dom + (c.free_surface ? c.pad : 2 * c.pad);
     p->nx = c.nx_dom + 2 * c.pad;
-    if (p->nz < 10 || p->nx < 10) {
-        delete p;
-        return fail(STFWI_ERR_INVALID, "padded grid %dx%d too small", p->nz, p->nx);
-    }
     p->z0 = c.free_surface ? 0 : c.pad;
     p->zrows = c.free_surface ? c.pad : 2 * c.pad;
     p->ld = ((p->nx + 8 + 63) / 64) * 64;
@@ -231,10 +240,11 @@ extern "C" int stfwi_plan_create(const stfwi_config *cfg, stfwi_plan **out)
     if (c.store_history && c.imaging) {
         p->ck = c.ckpt_every > 0 && c.ckpt_every < c.nt_max ? c.ckpt_every : 0;
         p->hist_nt = p->ck ? p->ck + 1 : c.nt_max;
-        rc |= dalloc(p, &p->hist, (size_t)S * p->hist_nt * 3 * p->np * es);
+        p->hist_cap = p->ck || c.hist_frames == 0 ? S * p->hist_nt : c.hist_frames;
+        rc |= dalloc(p, &p->hist, (size_t)p->hist_cap * 3 * p->np * es);
         if (c.grad_params == 3) {
             p->hist2_mode = c.imaging == 2 ? 1 : 2;
-            rc |= dalloc(p, &p->hist2, (size_t)S * p->hist_nt * 2 * p->np * es);
+            rc |= dalloc(p, &p->hist2, (size_t)p->hist_cap * 2 * p->np * es);
         }
         if (p->ck) {
             const i64 nslot = (c.nt_max + p->ck - 1) / p->ck;
@@ -261,6 +271,70 @@ extern "C" int stfwi_plan_create(const stfwi_config *cfg, stfwi_plan **out)
     rc |= dalloc(p, (void **)&p->lpart,
                  S * std::max(c.nrec_max, 1) * ((std::max(c.nt_max, 1) + RES_TCH - 1) / RES_TCH) * sizeof(double));
     rc |= dalloc(p, &p->injm, (size_t)S * c.nt_max * 4 * std::max(c.nrec_max, 1) * es);
+    return rc;
+}
+
+extern "C" int stfwi_plan_footprint(const stfwi_config *cfg, long long *bytes)
+{
+    if (!cfg || !bytes) return fail(STFWI_ERR_INVALID, "null argument");
+    const int v = validate_config(*cfg);
+    if (v) return v;
+    stfwi_plan p;
+    p.cfg = *cfg;
+    p.dry = true;
+    plan_alloc(&p);
+    *bytes = p.bytes;
+    return 0;
+}
+
+extern "C" int stfwi_plan_choose_ckpt(const stfwi_config *cfg, long long budget_bytes, int *ckpt_every)
+{
+    if (!cfg || !ckpt_every) return fail(STFWI_ERR_INVALID, "null argument");
+    stfwi_config c = *cfg;
+    c.ckpt_every = 0;
+    int v = validate_config(c);
+    if (v) return v;
+    long long full = 0;
+    stfwi_plan_footprint(&c, &full);
+    if (full <= budget_bytes || !c.store_history || !c.imaging) {
+        *ckpt_every = 0;
+        return 0;
+    }
+    if (c.imaging == STFWI_IMAGING_ZHOU && c.grad_params == STFWI_GRAD_ALL)
+        return fail(STFWI_ERR_UNSUPPORTED, "the history needs %lld bytes of %lld and Zhou imaging of rho cannot "
+                                           "be checkpointed", full, budget_bytes);
+    // footprint(K) = nslot(K) states + (K + 1) history frames, least near
+    // K = sqrt(nt * state bytes / frame bytes): scan every K, the set-up is
+    // host arithmetic only
+    long long best = -1;
+    int best_k = 0;
+    for (int k = 1; k < c.nt_max; ++k) {
+        c.ckpt_every = k;
+        long long b = 0;
+        stfwi_plan_footprint(&c, &b);
+        if (best < 0 || b < best) {
+            best = b;
+            best_k = k;
+        }
+    }
+    if (best < 0 || best > budget_bytes)
+        return fail(STFWI_ERR_OOM, "no checkpoint spacing fits: the smallest plan needs %lld bytes of %lld", best,
+                    budget_bytes);
+    *ckpt_every = best_k;
+    return 0;
+}
+
+extern "C" int stfwi_plan_create(const stfwi_config *cfg, stfwi_plan **out)
+{
+    if (!cfg || !out) return fail(STFWI_ERR_INVALID, "null argument");
+    *out = nullptr;
+    const int v = validate_config(*cfg);
+    if (v) return v;
+    const stfwi_config &c = *cfg;
+    stfwi_plan *p = new stfwi_plan();
+    p->cfg = c;
+    const int rc = plan_alloc(p);
+    const i64 S = c.n_shots_max;
     if (rc) {
         std::string e = g_err;
         stfwi_plan_destroy(p);
@@ -331,6 +405,18 @@ extern "C" int stfwi_set_shots(stfwi_plan *p, const stfwi_shot *shots, int n_sho
     std::vector<int> uptr(1, 0), urec;
     std::vector<int> rowcum((size_t)n_shots * (p->nz + 1), 0), shot_nt(n_shots);
     int nt_run = 0, max_nuc = 0, max_nrec = 0;
+    std::vector<int> hoff(n_shots, 0);
+    i64 frames = 0;
+    for (int s = 0; s < n_shots; ++s) {
+        // each shot's history holds its own record (or one checkpoint
+        // segment), packed one shot after another
+        const int nf = p->ck ? p->hist_nt : shots[s].nt;
+        if (p->hist && frames + std::max(nf, 0) > p->hist_cap)
+            return fail(STFWI_ERR_INVALID, "the shots' history needs more than the plan's %lld frames (shot %d)",
+                        (long long)p->hist_cap, s);
+        hoff[s] = (int)frames;
+        frames += std::max(nf, 0);
+    }
     for (int s = 0; s < n_shots; ++s) {
         const stfwi_shot &h = shots[s];
         if (h.src_type != STFWI_SRC_FORCE && h.src_type != STFWI_SRC_MOMENT)
@@ -345,6 +431,7 @@ extern "C" int stfwi_set_shots(stfwi_plan *p, const stfwi_shot *shots, int n_sho
         d.sz = h.src_z + p->z0;
         d.type = h.src_type;
         d.nt = h.nt;
+        d.hoff = hoff[s];
         d.rec_off = s * c.nrec_max;
         d.nrec = h.nrec;
         d.uc_off = (int)ucell.size();
@@ -395,6 +482,7 @@ extern "C" int stfwi_set_shots(stfwi_plan *p, const stfwi_shot *shots, int n_sho
     p->max_nuc = max_nuc;
     p->max_nrec = max_nrec;
     p->shot_nt = shot_nt;
+    p->shot_hoff = hoff;
     p->wav = p->wav_int;
     p->fwd_done = false;
     return 0;
@@ -778,7 +866,8 @@ static int adjoint_t(stfwi_plan *p, const void *rec_adj, void *src_adj, void *sn
                     }
                     const int nb = (int)std::min<i64>((p->np + 255) / 256, 1024);
                     hipLaunchKernelGGL(k_hist_from_state<T>, dim3(nb, ns), dim3(256), 0, gs[k], f.g,
-                                       (const T *)p->state, (i64)5 * p->fstride, (T *)p->hist, p->hist_nt, 0, s0);
+                                       (const T *)p->state, (i64)5 * p->fstride, (T *)p->hist,
+                                       (const ShotDev *)p->shots_d, 0, s0);
                 }
                 for (int it = t_lo; it < t_hi; ++it)
                     for (int k = 0; k < G; ++k) {
@@ -992,11 +1081,16 @@ static int time_kernel(stfwi_plan *p, int which, int tz, int reps, hipStream_t s
                    : time_kernel_t<float, 2>(p, which, tz, reps, st, ms, grouped);
 }
 
-extern "C" int stfwi_time_kernel(stfwi_plan *p, int which, int reps, void *stream, double *ms)
+extern "C" int stfwi_time_kernel(stfwi_plan *p, int which, int reps, int grouped, void *stream, double *ms)
 {
     if (!p || !ms || which < 0 || which > 3 || reps < 1) return fail(STFWI_ERR_INVALID, "bad argument");
     if (!p->have_model || !p->n_shots) return fail(STFWI_ERR_STATE, "model or shots not set");
-    return time_kernel(p, which, p->tz[which], reps, (hipStream_t)stream, ms);
+    const int rc = time_kernel(p, which, p->tz[which], reps, (hipStream_t)stream, ms, grouped != 0);
+    // the timing launches ran on the live buffers: a pending forward or
+    // adjoint is no longer valid
+    p->fwd_done = false;
+    p->adj_done = false;
+    return rc;
 }
 
 extern "C" int stfwi_autotune(stfwi_plan *p, int reps, void *stream, int *tz_out, double *us_out)
@@ -1350,7 +1444,12 @@ extern "C" int stfwi_bound_mask(const double *x, const double *g, const double *
 extern "C" int stfwi_set_graphs(stfwi_plan *p, int enable)
 {
     if (!p) return fail(STFWI_ERR_INVALID, "null plan");
-    p->graphs = enable != 0;
+    // Graph replay is refused: with it on, a GPU test run ended in a host
+    // segmentation fault inside stfwi_adjoint after the capture key changed,
+    // and no mistake in this library has been shown to cause it.  The eager
+    // time loops are the only path until one is.
+    if (enable) return fail(STFWI_ERR_UNSUPPORTED, "graph replay of the time loops is disabled");
+    p->graphs = false;
     return 0;
 }
 

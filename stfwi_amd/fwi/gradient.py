@@ -12,7 +12,10 @@
 Both backends run every shot in one batched launch: forward with the strain
 history, residual prep (taper, DF-I Butterworth cascade, L2 misfit and its
 transpose), and the adjoint with the imaging condition fused into each
-reverse step.  backend='adjoint' uses the Zhou imaging of
+reverse step.  When the strain history of all shots does not fit in the free
+device memory, the plan checkpoints the forward state and recomputes the
+history segment by segment (the library's planner picks the spacing; the
+results do not depend on it).  backend='adjoint' uses the Zhou imaging of
 fwi/imaging.py:83-169 (the reference's hand-coded gradient); backend='ad'
 uses the exact discrete gradient, i.e. what jax.grad returns in
 fwi/gradient.py:106-132.
@@ -78,7 +81,7 @@ def compute_gradient(backend: str, *, Vs, Vp, rho, src_x, src_z, src_wavelet, ob
     p = get_plan(nz_dom=int(nz_dom), nx_dom=int(nx_dom), dx=float(dx), dz=float(dz), dt=float(dt), fc=float(fc),
                  pad=int(pad), fd_order=fd, free_surface=bool(free_surface), dtype=dtype, n_shots_max=n,
                  nt_max=nt, nrec_max=nrec, imaging="zhou" if backend == "adjoint" else "exact", grad=invert,
-                 store_history=True, device=device())
+                 store_history=True, device=device(), ckpt_every="auto")
     sdt = storage_dtype(Vs)
     b_img = None
     if backend == "adjoint":

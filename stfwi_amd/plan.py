@@ -37,6 +37,22 @@ def padded_shape(nz_dom, nx_dom, pad, free_surface):
     return nz_dom + (pad if free_surface else 2 * pad), nx_dom + 2 * pad
 
 
+def footprint(cfg) -> int:
+    """Device bytes a plan of this _abi.Config would allocate (no allocation)."""
+    b = C.c_longlong()
+    check(lib().stfwi_plan_footprint(C.byref(cfg), C.byref(b)))
+    return int(b.value)
+
+
+def choose_ckpt(cfg, budget_bytes) -> int:
+    """Checkpoint spacing for a plan of this _abi.Config within budget_bytes:
+    0 when the whole history fits, else the K of the smallest footprint
+    (raises if none fits)."""
+    k = C.c_int()
+    check(lib().stfwi_plan_choose_ckpt(C.byref(cfg), int(budget_bytes), C.byref(k)))
+    return int(k.value)
+
+
 class Plan:
     """Device workspace for shots of one grid.
 
@@ -46,12 +62,16 @@ class Plan:
     'vs_vp', 'all' = the reference's ``invert``); ``store_history`` keeps the
     strain history for the imaging adjoint; ``ckpt_every`` = K > 0 keeps
     only forward-state checkpoints every K steps plus one K-step segment of
-    history and recomputes each segment during the adjoint (same results).
+    history and recomputes each segment during the adjoint (same results);
+    ``ckpt_every='auto'`` lets the library's planner pick K from the free
+    device memory (0 when the whole history fits).  ``hist_frames`` is the
+    history capacity in frames summed over the shots (each shot keeps its
+    own nt frames); 0 means n_shots_max * nt_max.
     """
 
     def __init__(self, nz_dom, nx_dom, dx, dz, dt, fc, pad, fd_order=2, free_surface=True,
                  dtype=torch.float32, n_shots_max=1, nt_max=1, nrec_max=1, imaging="none",
-                 grad="vs", store_history=False, device=None, ckpt_every=0):
+                 grad="vs", store_history=False, device=None, ckpt_every=0, hist_frames=0):
         if fd_order not in (2, 4, 8):
             raise NotImplementedError(f"fd_order={fd_order} not supported (use 2, 4, or 8).")
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
@@ -65,12 +85,17 @@ class Plan:
         self.nt_max, self.nrec_max, self.n_shots_max = int(nt_max), int(nrec_max), int(n_shots_max)
         self.imaging = imaging
         self.grad = grad
-        self.ckpt_every = int(ckpt_every)
         self.shape = padded_shape(self.nz_dom, self.nx_dom, self.pad, self.free_surface)
         cfg = _abi.Config(self.nz_dom, self.nx_dom, self.pad, int(self.free_surface), self.fd_order,
                           self.dx, self.dz, self.dt, self.fc, _code(dtype), self.n_shots_max,
                           self.nt_max, self.nrec_max, _abi.IMAGING[imaging], _abi.GRAD[grad],
-                          int(bool(store_history)), int(ckpt_every))
+                          int(bool(store_history)), 0, int(hist_frames))
+        if ckpt_every == "auto":
+            with torch.cuda.device(self.device):
+                free, _ = torch.cuda.mem_get_info()
+            ckpt_every = choose_ckpt(cfg, int(free * 0.9))
+        self.ckpt_every = int(ckpt_every)
+        cfg.ckpt_every = self.ckpt_every
         h = C.c_void_p()
         with torch.cuda.device(self.device):
             check(lib().stfwi_plan_create(C.byref(cfg), C.byref(h)))
@@ -246,8 +271,9 @@ class Plan:
         return list(tz), table
 
     def set_graphs(self, enable=True):
-        """Replay the forward / adjoint time loops as captured hipGraphs, or
-        launch them eagerly (the default)."""
+        """Graph replay of the time loops is refused by the library
+        (NotImplementedError for enable=True); set_graphs(False) keeps the
+        eager launches."""
         check(lib().stfwi_set_graphs(self._h, int(bool(enable))))
 
     def set_groups(self, groups):
@@ -264,12 +290,17 @@ class Plan:
         arr = (C.c_int * 4)(*[int(t) for t in tz])
         check(lib().stfwi_set_tiles(self._h, arr))
 
-    def time_kernel(self, which, reps=100):
-        """Average launch time in microseconds of one time-step kernel over
-        all shots (0 forward velocity, 1 forward strain, 2 adjoint strain,
-        3 adjoint velocity), measured with HIP events on the current stream."""
+    def time_kernel(self, which, reps=100, grouped=False):
+        """Average time in microseconds of one time-step kernel over all
+        shots (0 forward velocity, 1 forward strain, 2 adjoint strain,
+        3 adjoint velocity), measured with HIP events on the current stream.
+        grouped=True launches the plan's shot groups on their streams, as the
+        time loops do, and returns the time per step of all groups.  The
+        launches overwrite the wavefield state: run forward() again before
+        the next adjoint() or finalize()."""
         ms = C.c_double()
-        check(lib().stfwi_time_kernel(self._h, int(which), int(reps), _stream(), C.byref(ms)))
+        check(lib().stfwi_time_kernel(self._h, int(which), int(reps), int(bool(grouped)), _stream(),
+                                      C.byref(ms)))
         return ms.value * 1e3 / reps
 
     def finalize(self, vs, vp, rho):

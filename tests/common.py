@@ -72,3 +72,30 @@ def marmousi_init():
     for a, k in ((vs_i, "sha_Vs_init"), (vp_i, "sha_Vp_init"), (rho_i, "sha_rho_init")):
         assert hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest() == str(g[k]), k
     return dict(vs_true=vs_t, vp_true=vp_t, rho_true=rho_t, vs_init=vs_i, vp_init=vp_i, rho_init=rho_i)
+
+
+def zhou_constructed(nt=9, nz=6, nx=8, dx=10.0, dz=20.0, dt=1e-3):
+    """History cubes whose Zhou images are known in closed form.
+
+    Strains a, c, e are constant in time.  The adjoint velocities are
+    periodic in space, U_a = sin(kx j) and V_a = cos(kz i), so the wrapped
+    centred differences are exact: sin(kx) cos(kx j) / dx and
+    -sin(kz) sin(kz i) / dz.  For the density term the adjoint velocities
+    are t^2 h(z, x), t^2 h2(z, x) and the forward velocities t f, t f2, so
+    vx(t) (U_a(t+1) - U_a(t-1)) / (2 dt) = 2 t^2 f h / dt.  The buoyancy b is
+    constant."""
+    rng = np.random.default_rng(11)
+    t = np.arange(nt, dtype=np.float64)[:, None, None]
+    i = np.arange(nz)[None, :, None]
+    j = np.arange(nx)[None, None, :]
+    kx, kz = 2 * np.pi / nx, 2 * np.pi / nz
+    a, c, e = (rng.uniform(0.5, 1.5, (nz, nx)) for _ in range(3))
+    f, f2, h, h2 = (rng.standard_normal((nz, nx)) for _ in range(4))
+    shape = (nt, nz, nx)
+    cube = lambda x: np.broadcast_to(x, shape).copy()
+    return dict(nt=nt, dx=dx, dz=dz, dt=dt, b=4e-4, a=a, c=c, e=e, f=f, f2=f2, h=h, h2=h2,
+                ex=cube(a), ez=cube(c), es=cube(e), zero=np.zeros(shape),
+                sin_x=cube(np.sin(kx * j)), cos_z=cube(np.cos(kz * i)),
+                d_sin_x=np.sin(kx) * np.cos(kx * j[0]) / dx,      # d/dx sin(kx j), (1, nx)
+                d_cos_z=-np.sin(kz) * np.sin(kz * i[0]) / dz,     # d/dz cos(kz i), (nz, 1)
+                ua_t=t ** 2 * h, va_t=t ** 2 * h2, vx_t=t * f, vz_t=t * f2)

@@ -8,7 +8,8 @@ import torch
 import oracle
 from oracle import signal as osig
 from oracle import precond as opre
-from tests.common import (TOL_F64, dp_geometry, dp_model, dp_vectors, load_golden, marmousi_init, rel_l2)
+from tests.common import (TOL_F64, dp_geometry, dp_model, dp_vectors, load_golden, marmousi_init, rel_l2,
+                          zhou_constructed)
 
 pytestmark = pytest.mark.gpu
 
@@ -221,3 +222,35 @@ def test_run_lbfgs_dropins_decrease_misfit_and_save_models(tmp_path, x64):
     assert np.array_equal(p2[2], init[2])
     p3, hist3 = run_lbfgs(lf2, init, 2, save_dir=str(tmp_path / "c"), invert="vs_vp", early_stop_ratio=2.0)
     assert len(hist3) == 1        # ratio 1 < 2 stops after the first epoch
+
+
+def test_zhou_imaging_closed_forms(x64):
+    """imaging_condition (HIP, from cubes) on the constructed cubes whose
+    Zhou images are known in closed form: periodic centred differences of
+    B U_a and B V_a for g_lambda and g_mu, and the density term's sum over
+    t = 1 .. nt-2 (fwi/imaging.py:125-152).  The grid is the padded grid of
+    a 4 x 4 model with pad 2 and a free surface; rho is constant (b = 1/rho)."""
+    from stfwi_amd.fwi import imaging_condition
+    k = zhou_constructed()
+    nt, b, dt = k["nt"], k["b"], k["dt"]
+    pad, nzd, nxd = 2, 4, 4
+    rho = np.full((nzd, nxd), 1.0 / b)
+    vs, vp = np.full((nzd, nxd), 1000.0), np.full((nzd, nxd), 2000.0)
+    z = k["zero"]
+    crop = (slice(0, nzd), slice(pad, pad + nxd))
+
+    def run(ex, ez, es, ua, va, vx=None, vz=None):
+        out = imaging_condition(vs, vp, rho, ex, ez, es, ua, va, k["dx"], k["dz"], dt, pad, True, nzd, nxd,
+                                vx_hist=vx, vz_hist=vz)
+        return [_np(x) for x in out[3:]]
+
+    g_lam, g_mu, g_rl = run(k["ex"], k["ez"], z, k["sin_x"], k["cos_z"])
+    div = b * (k["d_sin_x"] + k["d_cos_z"])
+    assert rel_l2(g_lam, (-dt * nt * (k["a"] + k["c"]) * div)[crop]) < 1e-13
+    assert rel_l2(g_mu, (-dt * nt * 2 * b * (k["a"] * k["d_sin_x"] + k["c"] * k["d_cos_z"]))[crop]) < 1e-13
+    _, g_mu, _ = run(z, z, k["es"], k["cos_z"], k["sin_x"])
+    assert rel_l2(g_mu, (-dt * nt * k["e"] * b * (k["d_cos_z"] + k["d_sin_x"]))[crop]) < 1e-13
+    g_lam, g_mu, g_rl = run(z, z, z, k["ua_t"], k["va_t"], k["vx_t"], k["vz_t"])
+    assert not np.any(g_lam) and not np.any(g_mu)
+    closed = -2.0 * float(np.sum(np.arange(1, nt - 1) ** 2)) * (k["f"] * k["h"] + k["f2"] * k["h2"])
+    assert rel_l2(g_rl, closed[crop]) < 1e-13

@@ -38,7 +38,7 @@ def test_library_exports_every_declared_symbol(lib):
     # and the ctypes table binds exactly those
     from stfwi_amd._abi import SIGNATURES
     assert set(SIGNATURES) <= set(names)
-    assert lib.stfwi_abi_version() == 1
+    assert lib.stfwi_abi_version() == 2
 
 
 def _cfg(**kw):
@@ -54,7 +54,7 @@ def _cfg(**kw):
     (dict(fd_order=6), -2), (dict(fd_order=3), -2), (dict(dtype=7), -1), (dict(nt_max=0), -1),
     (dict(n_shots_max=0), -1), (dict(imaging=5), -1), (dict(grad_params=0), -1), (dict(dt=0.0), -1),
     (dict(nz_dom=1, nx_dom=1, pad=1), -1), (dict(ckpt_every=-1), -1),
-    (dict(ckpt_every=4, imaging=1, grad_params=3, store_history=1), -2),
+    (dict(ckpt_every=4, imaging=1, grad_params=3, store_history=1), -2), (dict(hist_frames=-1), -1),
 ])
 def test_plan_create_rejects_bad_config_without_device(lib, bad, code):
     h = C.c_void_p()
@@ -154,3 +154,38 @@ def test_marmousi_survey_matches_reference_fixtures(tmp_path):
     os.remove(tmp_path / "rho_true.npy")
     with pytest.raises(FileNotFoundError):
         load_marmousi(str(tmp_path))
+
+
+def test_plan_footprint_and_checkpoint_planner(lib):
+    """stfwi_plan_footprint sizes a plan without a device; the planner keeps
+    the whole history when it fits, otherwise picks the checkpoint spacing
+    of the smallest plan (near sqrt(nt)), and refuses what cannot fit."""
+    from stfwi_amd.plan import footprint, choose_ckpt
+    nz, nx, pad, nt, S = 152, 500, 20, 3000, 25
+    np_ = nz * nx
+    full = _cfg(nz_dom=nz, nx_dom=nx, pad=pad, n_shots_max=S, nt_max=nt, nrec_max=500, imaging=2,
+                store_history=1)
+    b_full = footprint(full)
+    hist = S * nt * 3 * np_ * 4
+    assert hist < b_full < hist * 1.1          # the strain history dominates
+    # per-shot history: the same plan with half the frames is smaller by them
+    half = _cfg(nz_dom=nz, nx_dom=nx, pad=pad, n_shots_max=S, nt_max=nt, nrec_max=500, imaging=2,
+                store_history=1, hist_frames=S * nt // 2)
+    assert b_full - footprint(half) == (S * nt - S * nt // 2) * 3 * np_ * 4
+    assert choose_ckpt(full, b_full) == 0
+    k = choose_ckpt(full, b_full // 4)
+    assert 20 <= k <= 200, k
+    kc = _cfg(nz_dom=nz, nx_dom=nx, pad=pad, n_shots_max=S, nt_max=nt, nrec_max=500, imaging=2,
+              store_history=1, ckpt_every=k)
+    b_k = footprint(kc)
+    assert b_k < b_full // 4
+    for k2 in (k // 2, 2 * k):           # the chosen spacing is the smallest plan
+        kc.ckpt_every = k2
+        assert footprint(kc) >= b_k
+    from stfwi_amd._abi import StfwiError
+    with pytest.raises(StfwiError):
+        choose_ckpt(full, 1 << 20)
+    zr = _cfg(nz_dom=nz, nx_dom=nx, pad=pad, n_shots_max=S, nt_max=nt, nrec_max=500, imaging=1, grad_params=3,
+              store_history=1)
+    with pytest.raises(NotImplementedError):
+        choose_ckpt(zr, footprint(zr) // 2)

@@ -9,7 +9,7 @@ import oracle
 from oracle import signal as osig
 from oracle import precond as opre
 from tests.common import (DP_CONFIGS, TOL_DOT, dp_geometry, dp_model, dp_vectors, load_golden,
-                          marmousi_init, rel_l2)
+                          marmousi_init, rel_l2, zhou_constructed)
 
 
 @pytest.mark.parametrize("fd,fs,st", DP_CONFIGS)
@@ -275,3 +275,62 @@ def test_oracle_source_taper_against_scipy_erf_and_geometry():
     assert np.all(np.diff(row) >= -1e-15)
     two = op.source_taper(40, 60, dh, [(10, 5), (50, 30)], R, filt_size=1)
     assert two[5, 10] == 0.0 and two[30, 50] == 0.0 and np.all((two >= 0) & (two <= 1))
+
+
+def test_zhou_cubes_closed_forms():
+    """oracle.zhou_cubes on constructed cubes against closed forms: the
+    periodic centred differences of the buoyancy-weighted adjoint velocities
+    (g_lambda, g_mu, the shear strain paired with dz(B U_a) + dx(B V_a)), and
+    the density term's sum over t = 1 .. nt-2 of vx(t) times the centred
+    time difference of U_a.  A sum over all t, a pairing shifted by one step
+    or the opposite sign each miss the closed form by far more than
+    rounding."""
+    k = zhou_constructed()
+    nt, b, dt = k["nt"], k["b"], k["dt"]
+    z = k["zero"]
+    # normal strains: g_lambda and g_mu
+    g_lam, g_mu, g_rl = oracle.zhou_cubes(k["ex"], k["ez"], z, k["sin_x"], k["cos_z"], b, k["dx"], k["dz"], dt)
+    div = b * (k["d_sin_x"] + k["d_cos_z"])
+    assert rel_l2(g_lam, -dt * nt * (k["a"] + k["c"]) * div) < 1e-13
+    assert rel_l2(g_mu, -dt * nt * 2 * b * (k["a"] * k["d_sin_x"] + k["c"] * k["d_cos_z"])) < 1e-13
+    assert not np.any(g_rl)
+    # shear strain alone: es (dz(B U_a) + dx(B V_a)), U_a periodic in z and
+    # V_a periodic in x
+    g_lam, g_mu, _ = oracle.zhou_cubes(z, z, k["es"], k["cos_z"], k["sin_x"], b, k["dx"], k["dz"], dt)
+    assert not np.any(g_lam)
+    assert rel_l2(g_mu, -dt * nt * k["e"] * b * (k["d_cos_z"] + k["d_sin_x"])) < 1e-13
+    # density term: strains zero, so only g_rho_lame
+    g_lam, g_mu, g_rl = oracle.zhou_cubes(z, z, z, k["ua_t"], k["va_t"], b, k["dx"], k["dz"], dt,
+                                          vx=k["vx_t"], vz=k["vz_t"])
+    assert not np.any(g_lam) and not np.any(g_mu)
+    fh = k["f"] * k["h"] + k["f2"] * k["h2"]
+    closed = -2.0 * float(np.sum(np.arange(1, nt - 1) ** 2)) * fh
+    assert rel_l2(g_rl, closed) < 1e-13
+    # the closed form separates the plausible mistakes
+    all_t = -2.0 * float(np.sum(np.arange(0, nt) ** 2)) * fh
+    shifted = -2.0 * float(np.sum(np.arange(1, nt - 1) * np.arange(2, nt))) * fh   # vx(t+1) with dU_a(t)
+    for wrong in (all_t, shifted, -closed):
+        assert rel_l2(g_rl, wrong) > 1e-2
+
+
+def test_zhou_fused_oracle_matches_cube_restatement():
+    """The C oracle fuses Zhou imaging into its adjoint (a three-frame ring of
+    carry-outs for the density term).  On its own forward and adjoint
+    history cubes it must equal zhou_cubes, the cube form of
+    fwi/imaging.py:111-152, for all three Lame images."""
+    m = dp_model()
+    for fd, fs, st in ((8, True, "moment"), (4, False, "force")):
+        sx, sz, rx, rz = dp_geometry(m["nz"], m["nx"], fs, st)
+        s, y = dp_vectors(rx.size, m["nt"])
+        g = oracle.Geom(m["nz"], m["nx"], m["pad"], fs, fd, m["nt"], m["dx"], m["dz"], m["dt"], m["fc"])
+        _, t = oracle.forward(g, m["Vs"], m["Vp"], m["rho"], sx, sz, s, rx, rz, st, np.float64, ("uv", "e"))
+        a = oracle.adjoint(g, m["Vs"], m["Vp"], m["rho"], sx, sz, y, rx, rz, st, s, "zhou", t,
+                           return_wavefields=True)
+        nz_t, nx_t = g.shape
+        rr = np.clip(np.arange(nz_t) - g.z0, 0, m["nz"] - 1)
+        cc = np.clip(np.arange(nx_t) - g.pad, 0, m["nx"] - 1)
+        b_full = 1.0 / m["rho"][rr][:, cc]
+        ref = oracle.zhou_cubes(t["e"][:, 0], t["e"][:, 1], t["e"][:, 2], a["ua"][:, 0], a["ua"][:, 1], b_full,
+                                m["dx"], m["dz"], m["dt"], vx=t["uv"][:, 0], vz=t["uv"][:, 1])
+        for i in range(3):
+            assert rel_l2(a["g"][i], ref[i]) < 1e-13, (fd, fs, st, i)
