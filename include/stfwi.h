@@ -196,6 +196,17 @@ int stfwi_eprecond3(const void *ws, int n_shots, int nz_t, int nx_t, int free_su
  * (eprecond1, fwi/preconditioning.py:37-62); dtype of ws, wr and we. */
 int stfwi_eprecond1(const void *ws, const void *wr, long long n, double eps, int dtype, void *we, void *stream);
 
+/* Allocation-free variant of stfwi_eprecond1 (no counterpart in the
+ * reference; for callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results, bit for bit.  Its reduction scratch (the
+ * partial maxima and the maximum, 1025 doubles = 8 KiB) is a file-scope
+ * device array of the library, one copy per GPU, held until the process
+ * ends; any n fits it.  Calls on one GPU share that copy: calls that
+ * overlap on one GPU (other streams or threads) race. */
+int stfwi_eprecond1_noalloc(const void *ws, const void *wr, long long n, double eps, int dtype, void *we,
+                            void *stream);
+
 /* Depth taper of the gradient, FP64 (nz x nx), rows as taper_grad
  * (fwi/preconditioning.py:144-183): 0 above gradt1, exp(-(gradt2-j)^2) up to
  * gradt2, (j dz)^exp_taper below; with bottom != 0 the roll-off between gradb1

@@ -46,6 +46,7 @@ SIGNATURES = {
     "stfwi_eprecond3": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _VP, _VP,
                                   C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _VP, _VP, _VP]),
     "stfwi_eprecond1": (C.c_int, [_VP, _VP, C.c_longlong, C.c_double, C.c_int, _VP, _VP]),
+    "stfwi_eprecond1_noalloc": (C.c_int, [_VP, _VP, C.c_longlong, C.c_double, C.c_int, _VP, _VP]),
     "stfwi_set_adjoint_energy": (C.c_int, [_VP, _VP, C.c_int]),
     "stfwi_taper_grad": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
                                    C.c_int, _VP, _VP]),

@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -1252,31 +1253,42 @@ extern "C" int stfwi_eprecond3(const void *ws, int n_shots, int nz_t, int nx_t, 
 // blocks of 256 threads for a grid-stride loop over n elements
 static int grid1(long long n) { return (int)std::min<long long>((n + 255) / 256, 8192); }
 
-// min (op = 0) or max (op = 1) of x[0, n) into *res: two passes through a
-// pool-allocated partials buffer
+// blocks of the first pass of a min/max reduction: at most MINMAX_PARTS
+// partials per call
+#define MINMAX_PARTS 1024
+
+// min (op = 0) or max (op = 1) of x[0, n) into *res: two passes through
+// `part` (MINMAX_PARTS elements)
 template <typename T>
-static int dev_minmax(const T *x, long long n, int op, T *res, hipStream_t st)
+static int dev_minmax_into(const T *x, long long n, int op, T *res, T *part, hipStream_t st)
 {
-    const int nb = (int)std::min<long long>((n + 255) / 256, 1024);
-    T *part = nullptr;
-    HIP_TRY(hipMallocAsync((void **)&part, nb * sizeof(T), st));
+    const int nb = (int)std::min<long long>((n + 255) / 256, MINMAX_PARTS);
     hipLaunchKernelGGL(k_minmax_part<T>, dim3(nb), dim3(256), 0, st, x, n, op, part);
     hipLaunchKernelGGL(k_minmax_part<T>, dim3(1), dim3(256), 0, st, (const T *)part, (long long)nb, op, res);
-    HIP_TRY(hipFreeAsync(part, st));
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
+// eprecond1 with its reduction scratch given: `part` holds MINMAX_PARTS
+// partial maxima and `mx` the maximum
+template <typename T>
+static int eprecond1_core(const T *ws, const T *wr, long long n, double eps, T *we, T *part, T *mx, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sqrt_prod<T>, dim3(grid1(n)), dim3(256), 0, st, ws, wr, n, we);
+    const int rc = dev_minmax_into<T>(we, n, 1, mx, part, st);
+    if (rc == 0) hipLaunchKernelGGL(k_add_eps_max<T>, dim3(grid1(n)), dim3(256), 0, st, we, n, (const T *)mx, eps);
+    HIP_TRY(hipGetLastError());
+    return rc;
+}
+
+// the scratch from the stream-ordered pool, freed after the kernels
 template <typename T>
 static int eprecond1_t(const T *ws, const T *wr, long long n, double eps, T *we, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_sqrt_prod<T>, dim3(grid1(n)), dim3(256), 0, st, ws, wr, n, we);
-    T *mx = nullptr;
-    HIP_TRY(hipMallocAsync((void **)&mx, sizeof(T), st));
-    const int rc = dev_minmax<T>(we, n, 1, mx, st);
-    if (rc == 0) hipLaunchKernelGGL(k_add_eps_max<T>, dim3(grid1(n)), dim3(256), 0, st, we, n, (const T *)mx, eps);
-    HIP_TRY(hipFreeAsync(mx, st));
-    HIP_TRY(hipGetLastError());
+    T *buf = nullptr;
+    HIP_TRY(hipMallocAsync((void **)&buf, (MINMAX_PARTS + 1) * sizeof(T), st));
+    const int rc = eprecond1_core<T>(ws, wr, n, eps, we, buf, buf + MINMAX_PARTS, st);
+    HIP_TRY(hipFreeAsync(buf, st));
     return rc;
 }
 
@@ -1287,6 +1299,64 @@ extern "C" int stfwi_eprecond1(const void *ws, const void *wr, long long n, doub
     if (dtype == STFWI_F64) return eprecond1_t<double>((const double *)ws, (const double *)wr, n, eps, (double *)we, st);
     if (dtype == STFWI_F32) return eprecond1_t<float>((const float *)ws, (const float *)wr, n, eps, (float *)we, st);
     return fail(STFWI_ERR_INVALID, "dtype=%d", dtype);
+}
+
+// ---------------------------------------------------------------------------
+// allocation-free eprecond1: the reduction scratch is a file-scope device
+// array, one copy per GPU, held until the process ends.  Its host-side name
+// is not a device address: host code uses only the per-device address from
+// hipGetSymbolAddress.  Every call on a GPU shares that GPU's copy, so calls
+// that overlap on one GPU (other streams, other threads) race.  Nothing in
+// it needs to start at zero: the first reduction pass writes every partial
+// the second reads, and the second writes the maximum.
+// ---------------------------------------------------------------------------
+#define NOALLOC_SCRATCH_DOUBLES (MINMAX_PARTS + 1)
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+#define STFWI_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[STFWI_MAX_DEVICES] = {};
+
+// the current device's copy of `scratch`, looked up on the first call there
+static int scratch_for_current_device(double **out)
+{
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= STFWI_MAX_DEVICES) return fail(STFWI_ERR_INVALID, "device %d outside [0, %d)", dev,
+                                                         STFWI_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        double *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(STFWI_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", dev,
+                        hipGetErrorString(e));
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+extern "C" int stfwi_eprecond1_noalloc(const void *ws, const void *wr, long long n, double eps, int dtype, void *we,
+                                       void *stream)
+{
+    if (!ws || !wr || !we || n < 1) return fail(STFWI_ERR_INVALID, "bad argument");
+    if (dtype != STFWI_F64 && dtype != STFWI_F32) return fail(STFWI_ERR_INVALID, "dtype=%d", dtype);
+    // the reduction needs min(ceil(n / 256), MINMAX_PARTS) partials plus the
+    // maximum: never more than the scratch holds, for any n
+    const long long need = std::min<long long>((n + 255) / 256, MINMAX_PARTS) + 1;
+    if (need > NOALLOC_SCRATCH_DOUBLES)
+        return fail(STFWI_ERR_INVALID, "n=%lld needs %lld scratch elements, more than the %d held", n, need,
+                    NOALLOC_SCRATCH_DOUBLES);
+    double *ptr = nullptr;
+    const int rc = scratch_for_current_device(&ptr);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    if (dtype == STFWI_F64)
+        return eprecond1_core<double>((const double *)ws, (const double *)wr, n, eps, (double *)we, ptr,
+                                      ptr + MINMAX_PARTS, st);
+    float *fp = (float *)ptr;
+    return eprecond1_core<float>((const float *)ws, (const float *)wr, n, eps, (float *)we, fp, fp + MINMAX_PARTS, st);
 }
 
 extern "C" int stfwi_taper_grad(int nz, int nx, double dz, int gradt1, int gradt2, double exp_taper, int bottom,
@@ -1312,14 +1382,14 @@ extern "C" int stfwi_source_taper(int nz, int nx, double dh, const int *src_xz, 
     double *patch = nullptr, *mm = nullptr;
     int *src = nullptr;
     HIP_TRY(hipMallocAsync((void **)&patch, (size_t)ps * ps * sizeof(double), st));
-    HIP_TRY(hipMallocAsync((void **)&mm, 2 * sizeof(double), st));
+    HIP_TRY(hipMallocAsync((void **)&mm, (2 + MINMAX_PARTS) * sizeof(double), st));
     HIP_TRY(hipMallocAsync((void **)&src, (size_t)std::max(nsrc, 1) * 2 * sizeof(int), st));
     if (nsrc > 0) HIP_TRY(hipMemcpyAsync(src, src_xz, (size_t)nsrc * 2 * sizeof(int), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_src_patch, dim3(1), dim3(256), 0, st, ps, dh, srt_radius, patch);
     hipLaunchKernelGGL(k_src_stamp, dim3(grid1(n)), dim3(256), 0, st, nz, nx, ps, (const double *)patch,
                        (const int *)src, nsrc, out);
-    int rc = dev_minmax<double>(out, n, 0, mm, st);
-    if (rc == 0) rc = dev_minmax<double>(out, n, 1, mm + 1, st);
+    int rc = dev_minmax_into<double>(out, n, 0, mm, mm + 2, st);
+    if (rc == 0) rc = dev_minmax_into<double>(out, n, 1, mm + 1, mm + 2, st);
     if (rc == 0)
         hipLaunchKernelGGL(k_src_finish, dim3(grid1(n)), dim3(256), 0, st, nz, nx, (const double *)mm,
                            (const int *)src, nsrc, filt_size, out);
