@@ -144,7 +144,7 @@ def learned_bench(reps=50):
     rng = np.random.default_rng(0)
     for _ in range(8):
         gr = rng.standard_normal((NZ, NX))
-        lp.add_pair(gr, np.ones((NZ, NX)), -gr)
+        lp.add_pair(gr, np.ones((NZ, NX)), np.full((NZ, NX), 2000.0), -gr)
     lp.train(epochs=1, lr=1e-3, batch=4)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -153,6 +153,111 @@ def learned_bench(reps=50):
     out["train_epoch_ms"] = round((time.perf_counter() - t0) / 5 * 1e3, 2)
     out["layer"] = f"batch {B}, {ci}->{co} channels, 3x3, {h}x{w}, BF16 operands, FP32 accumulation"
     return out
+
+
+def batched_bench(steps, warmup):
+    """64 uneven shots (record lengths 1500-4000 steps, 120-500 receivers,
+    force and moment sources, fd 4 and 8; stfwi_amd.surveys.batched_survey):
+    one batched launch per fd order, each shot's history sized by its own
+    record.  A step = the misfit gradient of all 64 shots (two plans, one per
+    fd order, run back to back), exact imaging of Vs, FP32."""
+    import torch
+    from stfwi_amd import Plan
+    from stfwi_amd.surveys import batched_survey, algorithmic_bytes
+    sv = batched_survey()
+    f32 = torch.float32
+    dev = torch.device("cuda", 0)
+    runs = []
+    for fd in (4, 8):
+        idx = [k for k in range(len(sv["src_x"])) if sv["fd_order"][k] == fd]
+        nts = [int(sv["nt"][k]) for k in idx]
+        p = Plan(sv["nz"], sv["nx"], sv["dx"], sv["dz"], sv["dt"], sv["fc"], sv["pad"], fd_order=fd,
+                 free_surface=True, dtype=f32, n_shots_max=len(idx), nt_max=max(nts), nrec_max=500,
+                 imaging="exact", grad="vs", store_history=True, hist_frames=sum(nts))
+        p.set_groups(3)
+
+        def shots(p=p, idx=idx, nts=nts):
+            p.set_shots([sv["src_x"][k] for k in idx], [sv["src_z"][k] for k in idx],
+                        [sv["src_type"][k] for k in idx], nts, [sv["rec_x"][k] for k in idx],
+                        [sv["rec_z"][k] for k in idx], sv["wavelet"])
+        p.set_model(*(torch.as_tensor(a, dtype=f32, device=dev) for a in sv["true"]))
+        shots()
+        obs = p.forward(channels=(2,))["rec"][:, 2].contiguous()
+        scale = float(1.0 / (torch.mean(obs.double() ** 2) + 1e-45))
+        init = tuple(torch.as_tensor(a, dtype=f32, device=dev) for a in sv["init"])
+        p.set_model(*init)
+        tiles, _ = p.autotune(10)
+        shots()
+        runs.append((p, obs, scale, init, len(idx), tiles))
+
+    def step():
+        for p, obs, scale, init, n, _ in runs:
+            p.set_model(*init)
+            rec = p.forward(channels=(2,))["rec"]
+            loss, radj = p.residual(rec, obs, 2, None, 0, scale, n)
+            p.adjoint(radj, want_src=False)
+            p.finalize(*init)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / steps
+    nshot = len(sv["src_x"])
+    byt = sum(algorithmic_bytes(sv["nz"], sv["nx"], sv["pad"], True, int(sv["nt"][k]), len(sv["rec_x"][k]))
+              for k in range(nshot))
+    return {"metric": "shot-gradients/s, 64 uneven shots at 152x500 (%HBM roofline)", "value": nshot / el,
+            "unit": "shot-gradients/s", "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": el * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded layered model, Ricker 8 Hz)",
+            "config": {"workload": "64-shot gradient, 152x500 free surface pad 20, nt 1500-4000 per shot, "
+                                   "120-500 receivers per shot, force and moment sources, fd 4 and 8, "
+                                   "exact imaging, Vs",
+                       "shots": nshot, "record_steps": int(sum(sv["nt"])), "grid": [sv["nz"], sv["nx"]]},
+            "path_roofline": {"algorithmic_GB_per_step": byt / 1e9, "achieved_GB_s": byt / el / 1e9,
+                              "frac_of_hbm_peak": byt / el / HBM_PEAK},
+            "tiles": {fd: r[5] for fd, r in zip((4, 8), runs)}}
+
+
+def learned_vs_plain(seeds, weights, mode):
+    """Time to the plain method's final Vs RMSE, learned-preconditioned
+    against plain L-BFGS, on held-out seeded layered models
+    (stfwi_amd.learned_eval; 25 moment shots, 152x500, nt=3000, FP32 'ad'
+    gradients; bands 3/5/8 Hz x 20 iterations).  Prints one JSON line."""
+    import torch
+    from stfwi_amd.learned import LearnedPreconditioner
+    from stfwi_amd import learned_eval as E
+    lp = LearnedPreconditioner.load(os.path.join(weights, f"{mode}.pt"))
+    rows = []
+    for seed in seeds:
+        r = E.evaluate_seed(seed, {mode: lp})
+        print(json.dumps(r), file=sys.stderr, flush=True)
+        rows.append(r)
+    ok = [r for r in rows if r["plain_improves"]]
+    reached = [r for r in ok if r["learned"][mode]["reached"]]
+    ratios = [r["learned"][mode]["time_ratio"] for r in reached]
+    plain_s = sum(r["plain_s"] for r in ok)
+    # a seed whose learned run never reaches the plain RMSE counts with its
+    # whole run time (a lower bound on its time to the target)
+    learned_s = sum(r["learned"][mode]["seconds"] if r["learned"][mode]["reached"] else r["learned"][mode]["run_s"]
+                    for r in ok)
+    return {"metric": "time to the plain method's final Vs RMSE, learned vs plain (ratio)",
+            "value": learned_s / plain_s if plain_s else None, "unit": "learned seconds / plain seconds",
+            "higher_is_better": False, "n_gpus": 1, "dtype": "f32", "data": "synthetic (seeded layered models)",
+            "config": {"workload": "25-shot 152x500 nt=3000 FWI, Vs only, bands 3/5/8 Hz x 20 L-BFGS iterations",
+                       "held_out_seeds": [int(s) for s in seeds], "target": mode, "weights": weights},
+            "seeds_where_plain_lowers_rmse": len(ok), "reached": len(reached),
+            "median_time_ratio_of_reached": float(np.median(ratios)) if ratios else None,
+            "plain_seconds": plain_s, "learned_seconds": learned_s,
+            "learned_steps": sum(r["learned"][mode]["learned_steps"] for r in ok),
+            "fallbacks": sum(r["learned"][mode]["fallbacks"] for r in ok),
+            "per_seed": [{"seed": r["seed"], "rmse0": round(r["rmse_initial"], 2),
+                          "plain_rmse": round(r["plain_rmse_final"], 2), "plain_s": round(r["plain_s"], 2),
+                          "learned_s": r["learned"][mode]["seconds"], "learned_rmse": round(
+                              r["learned"][mode]["rmse_final"], 2)} for r in rows]}
 
 
 def main():
@@ -168,6 +273,14 @@ def main():
     ap.add_argument("--groups", type=int, default=3, help="concurrent stream groups of shots (1-4)")
     ap.add_argument("--tiles", default=None,
                     help="fixed tile heights of the four time-step kernels, e.g. 8,8,8,4 (default: autotune)")
+    ap.add_argument("--workload", default="survey", choices=["survey", "batched"],
+                    help="survey: the 25-shot 152x500 nt=3000 gradient (default); batched: 64 uneven shots")
+    ap.add_argument("--learned-vs-plain", action="store_true",
+                    help="time to the plain method's final Vs RMSE, learned vs plain, on held-out seeds")
+    ap.add_argument("--seeds", default="2000-2015", help="held-out seeds for --learned-vs-plain (a-b)")
+    ap.add_argument("--weights", default=os.path.join(ROOT, "stfwi_amd", "weights"),
+                    help="directory of the trained preconditioner (<mode>.pt)")
+    ap.add_argument("--target", default="final", help="training target of the network to evaluate")
     ap.add_argument("--learned", action="store_true",
                     help="time the learned preconditioner's matrix-core convolution kernels and one "
                          "training epoch instead of the gradient path")
@@ -187,6 +300,13 @@ def main():
     torch.cuda.set_device(dev)
     if args.learned:
         print(json.dumps({"learned": learned_bench()}))
+        return
+    if args.learned_vs_plain:
+        a, b = (int(v) for v in args.seeds.split("-"))
+        print(json.dumps(learned_vs_plain(range(a, b + 1), args.weights, args.target)))
+        return
+    if args.workload == "batched":
+        print(json.dumps(batched_bench(args.steps, args.warmup)))
         return
     NSHOT = args.shots
     sv = survey(0)

@@ -40,3 +40,17 @@ def project_velocity(Vs, Vp, rho, K_margin: float = 0.04, vp_cap=None, vs_floor:
     vs = np.maximum(vs, float(vs_floor))
     rh = np.maximum(rh, float(rho_floor))
     return vs, vp, rh
+
+
+def project_velocity_device(vs, vp, rho, K_margin: float = 0.04, vp_cap=None, vs_floor: float = 1e-3,
+                            rho_floor: float = 1e-3):
+    """project_velocity on float64 device tensors (new tensors; the same
+    order of operations: Vp cap, then the Vs ceiling Vp / ratio, then the
+    floors)."""
+    import torch
+    vp = vp.to(torch.float64).clone()
+    if vp_cap is not None:
+        vp.clamp_(max=float(vp_cap))
+    vs = torch.minimum(vs.to(torch.float64), vp / k_safe_ratio(K_margin)).clamp_(min=float(vs_floor))
+    rh = rho.to(torch.float64).clamp(min=float(rho_floor))
+    return vs, vp, rh

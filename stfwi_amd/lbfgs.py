@@ -202,7 +202,8 @@ def minimize(fun: Callable, x0: torch.Tensor, lo: Optional[torch.Tensor] = None,
              maxiter: int = 50, m: int = 10, maxls: int = 20, c1: float = 1e-4, c2: float = 0.9,
              callback: Optional[Callable] = None, memory: Optional[LBFGSMemory] = None,
              direction_fn: Optional[Callable] = None, maxls_learned: int = 10, on_step: Optional[Callable] = None,
-             max_step: Optional[float] = None, fg0: Optional[tuple] = None, t_init: Optional[float] = None):
+             max_step: Optional[float] = None, fg0: Optional[tuple] = None, t_init: Optional[float] = None,
+             learned_step: str = "lbfgs"):
     """Minimise fun(x) -> (f, g) over float64 device vectors x, with optional
     per-entry bounds lo <= x <= hi (device tensors or None).
 
@@ -211,10 +212,12 @@ def minimize(fun: Callable, x0: torch.Tensor, lo: Optional[torch.Tensor] = None,
 
     direction_fn(x, g) -> d (or None) proposes a search direction, e.g. a
     learned preconditioner's.  It is tried first, scaled to the L-BFGS
-    direction's max |.|, with projected backtracking of at most
-    maxls_learned evaluations; when it is not a descent direction or finds no
-    decrease of f, the plain L-BFGS step is taken instead.  on_step(x_old,
-    g_old, x_new, g_new, learned) sees every accepted step.
+    direction's max |.| (learned_step='lbfgs') or taken as the step itself
+    (learned_step='own': the first trial is x + d), with projected
+    backtracking of at most maxls_learned evaluations; when it is not a
+    descent direction or finds no decrease of f, the plain L-BFGS step is
+    taken instead.  on_step(x_old, g_old, x_new, g_new, learned) sees every
+    accepted step.
 
     max_step caps the step length t of the line search (jaxopt's
     max_stepsize); fg0 = (f, g) at x0 saves the first evaluation when the
@@ -225,6 +228,8 @@ def minimize(fun: Callable, x0: torch.Tensor, lo: Optional[torch.Tensor] = None,
     (accepted learned steps), n_fallback (learned proposals rejected)."""
     if x0.dtype != torch.float64 or not x0.is_cuda:
         raise TypeError("x0 must be a float64 device tensor")
+    if learned_step not in ("lbfgs", "own"):
+        raise ValueError(f"learned_step={learned_step!r}; use 'lbfgs' or 'own'")
     x = x0.clone()
     if lo is not None or hi is not None:
         x = step_clip(x, torch.zeros_like(x), 0.0, lo, hi)
@@ -277,7 +282,8 @@ def minimize(fun: Callable, x0: torch.Tensor, lo: Optional[torch.Tensor] = None,
                 amax = float(dl.abs().max())
                 if dot(g, dl) < 0 and amax > 0:
                     ev = _Eval(fun, x, dl, lo, hi)
-                    r = _armijo_projected(ev, f, g, x, t0 * float(d.abs().max()) / amax, c1, maxls_learned)
+                    tl = 1.0 if learned_step == "own" else t0 * float(d.abs().max()) / amax
+                    r = _armijo_projected(ev, f, g, x, tl, c1, maxls_learned)
                     nfev += ev.nfev
                     if r is not None and r[1] < f:
                         res, learned = r, True

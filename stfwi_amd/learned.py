@@ -1,11 +1,12 @@
 """Learned gradient preconditioner for the plain L-BFGS method.
 
-A small U-Net maps the raw Vs gradient and the illumination map of an FWI
-iteration to a search direction.  It is trained on (gradient, illumination)
--> accepted model update pairs that the plain method produces, and the
-driver falls back to the plain L-BFGS step whenever the learned direction
-does not decrease the misfit.  The reference has no learned part; this module
-has no reference counterpart.
+A U-Net maps three maps of an FWI iteration -- the raw Vs gradient, the
+illumination and the current Vs model -- to a search direction.  It is
+trained on pairs that the plain method produces (the inputs at an iterate
+and the model update the plain method went on to accept), and the driver
+falls back to the plain L-BFGS step whenever the learned direction does not
+decrease the misfit.  The reference has no learned part; this module has no
+reference counterpart.
 
 Every convolution runs on the matrix cores through hand-written HIP kernels
 (``stfwi_conv2d_bf16`` and its data/weight gradients: BF16 operands, FP32
@@ -13,10 +14,14 @@ accumulation); pooling, upsampling, the ReLU gradient, the MSE loss and Adam
 are HIP kernels too.  Parameters and activations are FP32 device tensors;
 torch only allocates and copies them.
 
-Architecture (c1 = 16, c2 = 32 channels by default):
-    x -> conv3 c1 -> conv3 c1 = e1 -> pool2 -> conv3 c2 -> conv3 c2 = e2
-      -> up2, concat [e1, up(e2)] -> conv3 c1 -> conv3 c1 -> conv1 -> y
-with ReLU after every 3x3 convolution.
+Architecture, for channels (c0, .., c{L-1}) = (16, 32, 64, 128) by default:
+    encoder level l: conv3 -> c_l, conv3 c_l -> c_l = e_l, then 2x2 average
+        pooling into level l + 1 (not after the last level);
+    decoder level l = L-2 .. 0: nearest 2x upsampling of the level below,
+        concatenated after e_l, conv3 -> c_l, conv3 c_l -> c_l;
+    a 1x1 convolution to the output;
+with ReLU after every 3x3 convolution.  H and W must be multiples of
+2^(L-1); the preconditioner pads the grid to that.
 """
 from __future__ import annotations
 
@@ -129,18 +134,23 @@ def mse(y, t, wgt=None):
 # ------------------------------------------------------------------ U-Net --
 
 class UNet:
-    SPECS = (("e1a", "in", "c1", 3), ("e1b", "c1", "c1", 3), ("e2a", "c1", "c2", 3), ("e2b", "c2", "c2", 3),
-             ("d1a", "c12", "c1", 3), ("d1b", "c1", "c1", 3), ("out", "c1", "out", 1))
-
-    def __init__(self, in_ch=2, out_ch=1, c1=16, c2=32, seed=0, device=None):
+    def __init__(self, in_ch=3, out_ch=1, channels=(16, 32, 64, 128), seed=0, device=None):
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
-        dims = {"in": in_ch, "out": out_ch, "c1": c1, "c2": c2, "c12": c1 + c2}
-        self.c1, self.c2 = c1, c2
+        self.channels = tuple(int(c) for c in channels)
+        L = len(self.channels)
+        specs, prev = [], in_ch
+        for l, c in enumerate(self.channels):
+            specs += [(f"enc{l}a", prev, c, 3), (f"enc{l}b", c, c, 3)]
+            prev = c
+        for l in range(L - 2, -1, -1):
+            c = self.channels[l]
+            specs += [(f"dec{l}a", c + self.channels[l + 1], c, 3), (f"dec{l}b", c, c, 3)]
+        specs.append(("out", self.channels[0], out_ch, 1))
+        self.specs = specs
         g = torch.Generator().manual_seed(seed)
         self.params = {}
-        for name, ci, co, k in self.SPECS:
-            cin, cout = dims[ci], dims[co]
-            std = math.sqrt(2.0 / (cin * k * k)) if name != "out" else math.sqrt(1.0 / (cin * k * k))
+        for name, cin, cout, k in specs:
+            std = math.sqrt((1.0 if name == "out" else 2.0) / (cin * k * k))
             w = (torch.randn((cout, cin, k, k), generator=g) * std).to(self.device)
             self.params[name] = [w, torch.zeros(cout, device=self.device)]
         self.adam = {n: [[torch.zeros_like(t) for t in ps], [torch.zeros_like(t) for t in ps]]
@@ -148,51 +158,73 @@ class UNet:
         self.step_count = 0
         self._cache = None
 
+    @property
+    def multiple(self):
+        return 1 << (len(self.channels) - 1)
+
+    def n_params(self):
+        return sum(t.numel() for ps in self.params.values() for t in ps)
+
     # forward -------------------------------------------------------------
     def forward(self, x, keep=True):
-        """x: (B, in_ch, H, W) FP32, H and W even."""
-        P = self.params
+        """x: (B, in_ch, H, W) FP32, H and W multiples of 2^(L-1)."""
+        P, C, L = self.params, self.channels, len(self.channels)
         B, _, H, W = x.shape
-        if H % 2 or W % 2:
-            raise ValueError("the U-Net needs even H and W (pad the input)")
-        e1a = conv(x, *P["e1a"], relu=True)
-        e1 = conv(e1a, *P["e1b"], relu=True)
-        p1 = pool2(e1)
-        e2a = conv(p1, *P["e2a"], relu=True)
-        e2 = conv(e2a, *P["e2b"], relu=True)
-        cat = torch.empty((B, self.c1 + self.c2, H, W), dtype=torch.float32, device=x.device)
-        cat[:, :self.c1].copy_(e1)
-        up2_into(e2, cat, self.c1)
-        d1a = conv(cat, *P["d1a"], relu=True)
-        d1 = conv(d1a, *P["d1b"], relu=True)
-        y = conv(d1, *P["out"], relu=False)
+        if H % self.multiple or W % self.multiple:
+            raise ValueError(f"the U-Net needs H and W divisible by {self.multiple} (pad the input)")
+        enc = []                      # per level: (input, first conv, second conv = skip)
+        h = x
+        for l in range(L):
+            a = conv(h, *P[f"enc{l}a"], relu=True)
+            e = conv(a, *P[f"enc{l}b"], relu=True)
+            enc.append((h, a, e))
+            if l < L - 1:
+                h = pool2(e)
+        dec = {}                      # per level: (concat, first conv, second conv)
+        d = enc[-1][2]
+        for l in range(L - 2, -1, -1):
+            skip = enc[l][2]
+            cat = torch.empty((B, C[l] + C[l + 1], skip.shape[2], skip.shape[3]), dtype=torch.float32,
+                              device=x.device)
+            cat[:, :C[l]].copy_(skip)
+            up2_into(d, cat, C[l])
+            a = conv(cat, *P[f"dec{l}a"], relu=True)
+            d = conv(a, *P[f"dec{l}b"], relu=True)
+            dec[l] = (cat, a, d)
+        y = conv(d, *P["out"], relu=False)
         if keep:
-            self._cache = (x, e1a, e1, p1, e2a, e2, cat, d1a, d1)
+            self._cache = (enc, dec, d)
         return y
 
     # backward ------------------------------------------------------------
     def backward(self, gy):
         """Parameter gradients {name: [dw, db]} of the last forward."""
-        x, e1a, e1, p1, e2a, e2, cat, d1a, d1 = self._cache
-        P = self.params
+        enc, dec, d_top = self._cache
+        P, C, L = self.params, self.channels, len(self.channels)
         G = {}
-        G["out"] = list(conv_wgrad(d1, gy, P["out"][0].shape))
-        g = relu_bwd_(conv_dgrad(gy, P["out"][0]), d1)
-        G["d1b"] = list(conv_wgrad(d1a, g, P["d1b"][0].shape))
-        g = relu_bwd_(conv_dgrad(g, P["d1b"][0]), d1a)
-        G["d1a"] = list(conv_wgrad(cat, g, P["d1a"][0].shape))
-        gcat = conv_dgrad(g, P["d1a"][0])
-        h, w = e2.shape[2], e2.shape[3]
-        g2 = relu_bwd_(up2_bwd_from(gcat, self.c1, self.c2, h, w), e2)
-        G["e2b"] = list(conv_wgrad(e2a, g2, P["e2b"][0].shape))
-        g2 = relu_bwd_(conv_dgrad(g2, P["e2b"][0]), e2a)
-        G["e2a"] = list(conv_wgrad(p1, g2, P["e2a"][0].shape))
-        g1 = pool2_bwd(conv_dgrad(g2, P["e2a"][0]), e1.shape)
-        g1 += gcat[:, :self.c1]
-        relu_bwd_(g1, e1)
-        G["e1b"] = list(conv_wgrad(e1a, g1, P["e1b"][0].shape))
-        g1 = relu_bwd_(conv_dgrad(g1, P["e1b"][0]), e1a)
-        G["e1a"] = list(conv_wgrad(x, g1, P["e1a"][0].shape))
+        G["out"] = list(conv_wgrad(d_top, gy, P["out"][0].shape))
+        g = relu_bwd_(conv_dgrad(gy, P["out"][0]), d_top)       # w.r.t. the last decoder output
+        skip_grad = {}
+        for l in range(0, L - 1):
+            cat, a, d = dec[l]
+            G[f"dec{l}b"] = list(conv_wgrad(a, g, P[f"dec{l}b"][0].shape))
+            g = relu_bwd_(conv_dgrad(g, P[f"dec{l}b"][0]), a)
+            G[f"dec{l}a"] = list(conv_wgrad(cat, g, P[f"dec{l}a"][0].shape))
+            gcat = conv_dgrad(g, P[f"dec{l}a"][0])
+            skip_grad[l] = gcat[:, :C[l]]
+            below = dec[l + 1][2] if l + 1 < L - 1 else enc[L - 1][2]
+            g = relu_bwd_(up2_bwd_from(gcat, C[l], C[l + 1], below.shape[2], below.shape[3]), below)
+        # g: gradient w.r.t. the deepest encoder output, ReLU applied
+        for l in range(L - 1, -1, -1):
+            h, a, e = enc[l]
+            if l < L - 1:
+                g = g + skip_grad[l]
+                relu_bwd_(g, e)
+            G[f"enc{l}b"] = list(conv_wgrad(a, g, P[f"enc{l}b"][0].shape))
+            g = relu_bwd_(conv_dgrad(g, P[f"enc{l}b"][0]), a)
+            G[f"enc{l}a"] = list(conv_wgrad(h, g, P[f"enc{l}a"][0].shape))
+            if l > 0:
+                g = pool2_bwd(conv_dgrad(g, P[f"enc{l}a"][0]), enc[l - 1][2].shape)
         return G
 
     def adam_step(self, grads, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
@@ -214,55 +246,74 @@ class UNet:
 
 # ----------------------------------------------------- the preconditioner --
 
-class LearnedPreconditioner:
-    """(Vs gradient, illumination) -> Vs search direction on an (nz, nx)
-    physical grid.  Inputs are normalised by their max |.|; the output is a
-    unit-free direction the line search scales.  Training targets are the
-    plain method's accepted Vs updates, normalised the same way."""
+def _unit(t):
+    return t / t.abs().max().clamp_min(1e-30)
 
-    def __init__(self, nz, nx, c1=16, c2=32, seed=0, device=None):
+
+class LearnedPreconditioner:
+    """(Vs gradient, illumination, Vs) -> Vs update on an (nz, nx) physical
+    grid.  The gradient and the illumination enter divided by their largest
+    |value|, the model divided by `vs_ref` m/s.  The network predicts the
+    update in units of `step_ref` m/s, so its output carries a step length
+    as well as a direction (``absolute``: the driver tries it as the step
+    itself).  Training targets are model updates divided by `step_ref`."""
+
+    absolute = True
+
+    def __init__(self, nz, nx, channels=(16, 32, 64, 128), seed=0, device=None, vs_ref=3000.0, step_ref=100.0):
         self.nz, self.nx = int(nz), int(nx)
-        self.H, self.W = self.nz + self.nz % 2, self.nx + self.nx % 2
-        self.net = UNet(2, 1, c1, c2, seed, device)
+        self.net = UNet(3, 1, channels, seed, device)
+        m = self.net.multiple
+        self.H, self.W = -(-self.nz // m) * m, -(-self.nx // m) * m
         self.device = self.net.device
+        self.vs_ref = float(vs_ref)
+        self.step_ref = float(step_ref)
         self.pairs: list = []
 
-    def _features(self, grad, illum):
-        x = torch.zeros((1, 2, self.H, self.W), dtype=torch.float32, device=self.device)
+    def features(self, grad, illum, vs):
+        x = torch.zeros((1, 3, self.H, self.W), dtype=torch.float32, device=self.device)
         g = torch.as_tensor(grad, device=self.device).float().reshape(self.nz, self.nx)
         il = torch.ones_like(g) if illum is None else torch.as_tensor(illum, device=self.device).float().reshape(
             self.nz, self.nx)
-        x[0, 0, :self.nz, :self.nx] = g / g.abs().max().clamp_min(1e-30)
-        x[0, 1, :self.nz, :self.nx] = il / il.abs().max().clamp_min(1e-30)
+        v = torch.as_tensor(vs, device=self.device).float().reshape(self.nz, self.nx)
+        x[0, 0, :self.nz, :self.nx] = _unit(g)
+        x[0, 1, :self.nz, :self.nx] = _unit(il)
+        x[0, 2, :self.nz, :self.nx] = v / self.vs_ref
         return x
 
-    def direction(self, grad, illum=None):
-        """Predicted Vs update direction (nz, nx), float64."""
-        y = self.net.forward(self._features(grad, illum), keep=False)
-        return y[0, 0, :self.nz, :self.nx].double()
-
-    def add_pair(self, grad, illum, step):
-        s = torch.as_tensor(step, device=self.device).float().reshape(self.nz, self.nx)
+    def target(self, step):
         t = torch.zeros((1, 1, self.H, self.W), dtype=torch.float32, device=self.device)
-        t[0, 0, :self.nz, :self.nx] = s / s.abs().max().clamp_min(1e-30)
-        self.pairs.append((self._features(grad, illum), t))
+        t[0, 0, :self.nz, :self.nx] = torch.as_tensor(step, device=self.device).float().reshape(
+            self.nz, self.nx) / self.step_ref
+        return t
 
-    def train(self, epochs=200, lr=1e-3, batch=4, seed=0):
-        """Fit the stored pairs (MSE over the physical cells); returns the
-        loss history (one value per epoch, mean over batches)."""
-        if not self.pairs:
+    def direction(self, grad, illum=None, vs=None):
+        """Predicted Vs update (nz, nx) in m/s, float64."""
+        if vs is None:
+            vs = torch.full((self.nz, self.nx), self.vs_ref, device=self.device)
+        y = self.net.forward(self.features(grad, illum, vs), keep=False)
+        return y[0, 0, :self.nz, :self.nx].double() * self.step_ref
+
+    def add_pair(self, grad, illum, vs, step):
+        self.pairs.append((self.features(grad, illum, vs), self.target(step)))
+
+    def train(self, epochs=200, lr=1e-3, batch=4, seed=0, pairs=None):
+        """Fit the pairs (MSE over the physical cells); returns the loss
+        history (one value per epoch, mean over batches)."""
+        pairs = self.pairs if pairs is None else pairs
+        if not pairs:
             return []
         rng = np.random.default_rng(seed)
         wgt = torch.zeros((1, 1, self.H, self.W), dtype=torch.float32, device=self.device)
         wgt[0, 0, :self.nz, :self.nx] = 1.0
         hist = []
         for _ in range(epochs):
-            order = rng.permutation(len(self.pairs))
+            order = rng.permutation(len(pairs))
             tot, nb = 0.0, 0
             for i in range(0, len(order), batch):
                 idx = order[i:i + batch]
-                x = torch.cat([self.pairs[j][0] for j in idx])
-                t = torch.cat([self.pairs[j][1] for j in idx])
+                x = torch.cat([pairs[j][0] for j in idx])
+                t = torch.cat([pairs[j][1] for j in idx])
                 y = self.net.forward(x)
                 loss, gy = mse(y, t, wgt.expand_as(t).contiguous())
                 self.net.adam_step(self.net.backward(gy), lr=lr)
@@ -270,3 +321,53 @@ class LearnedPreconditioner:
                 nb += 1
             hist.append(tot / nb)
         return hist
+
+    def save(self, path):
+        torch.save({"state": self.net.state_dict(), "channels": self.net.channels, "nz": self.nz, "nx": self.nx,
+                    "vs_ref": self.vs_ref, "step_ref": self.step_ref}, path)
+
+    @classmethod
+    def load(cls, path, device=None):
+        d = torch.load(path, map_location="cpu", weights_only=True)
+        lp = cls(d["nz"], d["nx"], tuple(d["channels"]), device=device, vs_ref=d["vs_ref"], step_ref=d["step_ref"])
+        lp.net.load_state_dict(d["state"])
+        return lp
+
+
+class PairCollector:
+    """Training pairs from plain-method runs (run_multiscale_fwi(collect=)).
+
+    Every accepted step of a Vs-only stage gives the inputs at its starting
+    iterate.  Each pair receives three targets: the accepted step itself
+    ('step'), the update from its iterate to the stage's final model
+    ('ahead', the rest of the plain stage in one step) and the update to the
+    run's final model ('final', the rest of the plain run)."""
+
+    MODES = ("step", "ahead", "final")
+
+    def __init__(self, lp: LearnedPreconditioner):
+        self.lp = lp
+        self.stage_open: list = []
+        self.run_open: list = []
+        self.pairs = {m: [] for m in self.MODES}
+
+    def add_step(self, grad, illum, vs, step):
+        self.stage_open.append((self.lp.features(grad, illum, vs), self.lp.target(step),
+                                torch.as_tensor(vs, device=self.lp.device).detach().clone()))
+
+    def end_stage(self, vs_final):
+        vf = torch.as_tensor(vs_final, device=self.lp.device)
+        for x, t_step, vs in self.stage_open:
+            self.pairs["step"].append((x, t_step))
+            self.pairs["ahead"].append((x, self.lp.target(vf - vs)))
+            self.run_open.append((x, vs))
+        self.stage_open = []
+
+    def end_run(self, vs_final):
+        vf = torch.as_tensor(vs_final, device=self.lp.device)
+        for x, vs in self.run_open:
+            self.pairs["final"].append((x, self.lp.target(vf - vs)))
+        self.run_open = []
+
+    def __len__(self):
+        return len(self.pairs["step"])

@@ -254,3 +254,34 @@ def test_zhou_imaging_closed_forms(x64):
     assert not np.any(g_lam) and not np.any(g_mu)
     closed = -2.0 * float(np.sum(np.arange(1, nt - 1) ** 2)) * (k["f"] * k["h"] + k["f2"] * k["h2"])
     assert rel_l2(g_rl, closed[crop]) < 1e-13
+
+
+def test_build_loss_fn_tikhonov_composition(x64):
+    """fwi/loss.py:123-130: total = misfit + alpha * stop_grad(misfit / reg) * reg.
+    The value is exactly (1 + alpha) times the misfit, and the gradient adds
+    alpha * (misfit / reg) times the regulariser's gradient, the ratio held
+    fixed."""
+    from stfwi_amd.fwi import build_forward_fn, build_loss_fn
+    from stfwi_amd.fwi.loss import _tikhonov
+    m = dp_model()
+    fs, fd = True, 8
+    rx = np.arange(2, m["nx"] - 2, 3)
+    rz = np.zeros_like(rx)
+    s, _ = dp_vectors(rx.size, m["nt"])
+    run = build_forward_fn(m["nz"], m["nx"], m["dx"], m["dz"], m["dt"], m["nt"], m["fc"], m["pad"], 60, rx, rz,
+                           fd_order=fd, free_surface=fs)
+    sx, sz = np.array([6, 20]), np.array([0, 0])
+    obs = run.batched(m["Vs"], m["Vp"], m["rho"], sx, sz, s)[2]
+    params = (m["Vs"] * 1.02, m["Vp"], m["rho"])
+    alpha = 0.01
+    for invert in ("vs", "all"):
+        plain = build_loss_fn(run, obs, sx, sz, s, component=2, scale=1e15, invert=invert)
+        tik = build_loss_fn(run, obs, sx, sz, s, component=2, scale=1e15, invert=invert, tikhonov_alpha=alpha)
+        l0, g0 = plain(params)
+        l1, g1 = tik(params)
+        assert abs(l1 / ((1 + alpha) * l0) - 1) < 1e-14
+        reg, rg = _tikhonov([torch.as_tensor(p, dtype=torch.float64).cuda() for p in params], invert)
+        w = alpha * l0 / (reg + 1e-30)
+        for i in range(3):
+            expect = _np(g0[i]) + w * _np(rg[i])
+            assert rel_l2(_np(g1[i]), expect) < 1e-13 or not np.any(expect), (invert, i)

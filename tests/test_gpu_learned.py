@@ -111,27 +111,40 @@ class _BfConv(torch.autograd.Function):
         return gx, gw, gy.sum(dim=(0, 2, 3)), None
 
 
-def _cpu_unet(params, x):
+def _cpu_unet(params, x, channels):
     c = lambda t, n, relu=True: (torch.relu if relu else (lambda z: z))(
         _BfConv.apply(t, params[n][0], params[n][1], params[n][0].shape[-1]))
-    e1 = c(c(x, "e1a"), "e1b")
-    e2 = c(c(F.avg_pool2d(e1, 2), "e2a"), "e2b")
-    cat = torch.cat([e1, F.interpolate(e2, scale_factor=2, mode="nearest")], dim=1)
-    return c(c(c(cat, "d1a"), "d1b"), "out", relu=False)
+    L = len(channels)
+    skips, h = [], x
+    for l in range(L):
+        e = c(c(h, f"enc{l}a"), f"enc{l}b")
+        skips.append(e)
+        if l < L - 1:
+            h = F.avg_pool2d(e, 2)
+    d = skips[-1]
+    for l in range(L - 2, -1, -1):
+        cat = torch.cat([skips[l], F.interpolate(d, scale_factor=2, mode="nearest")], dim=1)
+        d = c(c(cat, f"dec{l}a"), f"dec{l}b")
+    return c(d, "out", relu=False)
 
 
-def test_unet_forward_backward_match_cpu_reference():
+@pytest.mark.parametrize("channels", [(16, 32, 64, 128), (8, 16)])
+def test_unet_forward_backward_match_cpu_reference(channels):
+    """The 4-level 16/32/64/128 U-Net with 3 inputs (and a 2-level one)
+    against a CPU autograd U-Net of the same BF16 rounding: outputs, loss
+    and every parameter gradient."""
     from stfwi_amd.learned import UNet, mse
-    net = UNet(2, 1, 16, 32, seed=3)
+    net = UNet(3, 1, channels, seed=3)
     g = torch.Generator().manual_seed(4)
-    x = torch.randn((2, 2, 24, 36), generator=g)
-    t = torch.randn((2, 1, 24, 36), generator=g)
+    x = torch.randn((2, 3, 24, 40), generator=g)
+    t = torch.randn((2, 1, 24, 40), generator=g)
     y = net.forward(x.cuda())
     loss, gy = mse(y, t.cuda())
     grads = net.backward(gy)
+    assert set(grads) == set(net.params) and len(net.params) == 4 * len(channels) - 1
     params = {n: [ps[0].cpu().double().requires_grad_(True), ps[1].cpu().double().requires_grad_(True)]
               for n, ps in net.params.items()}
-    yr = _cpu_unet(params, x.double())
+    yr = _cpu_unet(params, x.double(), channels)
     lr = ((yr - t.double()) ** 2).mean()
     lr.backward()
     assert _rel(y.cpu(), yr.detach()) < 1e-3
@@ -147,19 +160,39 @@ def test_training_reduces_loss_and_preconditioner_round_trip():
     from stfwi_amd.learned import LearnedPreconditioner
     nz, nx = 30, 50
     lp = LearnedPreconditioner(nz, nx, seed=1)
+    assert (lp.H, lp.W) == (32, 56)          # padded to multiples of 8
     rng = np.random.default_rng(0)
     zz, xx = np.meshgrid(np.arange(nz), np.arange(nx), indexing="ij")
+    vs = 1500.0 + 20.0 * zz
     for i in range(8):
         gr = rng.standard_normal((nz, nx))
         il = 1.0 / (1.0 + 0.1 * zz + 0.01 * xx * (i + 1))
         step = -(gr * il)
-        lp.add_pair(gr, il, step)
+        lp.add_pair(gr, il, vs, step)
     hist = lp.train(epochs=150, lr=3e-3, batch=4)
     assert hist[-1] < 0.1 * hist[0], (hist[0], hist[-1])
-    d = lp.direction(rng.standard_normal((nz, nx)), il)
+    d = lp.direction(rng.standard_normal((nz, nx)), il, vs)
     assert d.shape == (nz, nx) and d.dtype == torch.float64 and torch.isfinite(d).all()
-    sd = lp.net.state_dict()
     lp2 = LearnedPreconditioner(nz, nx, seed=9)
-    lp2.net.load_state_dict(sd)
+    lp2.net.load_state_dict(lp.net.state_dict())
     gtest = rng.standard_normal((nz, nx))
-    assert torch.equal(lp.direction(gtest, il), lp2.direction(gtest, il))
+    assert torch.equal(lp.direction(gtest, il, vs), lp2.direction(gtest, il, vs))
+
+
+def test_learned_vs_plain_evaluation_runs_end_to_end():
+    """The learned-vs-plain evaluation path on the 152x500, nt=3000 survey of
+    one held-out seed, cut to 2 + 1 + 1 iterations: the plain run, then the
+    learned run stopped at the plain final RMSE, with the trained network
+    when it is in the tree (stfwi_amd/weights), else an untrained one."""
+    import os
+    from stfwi_amd import learned_eval as E
+    from stfwi_amd.learned import LearnedPreconditioner
+    w = os.path.join(os.path.dirname(E.__file__), "weights", "final.pt")
+    lp = LearnedPreconditioner.load(w) if os.path.exists(w) else LearnedPreconditioner(152, 500, seed=0)
+    r = E.evaluate_seed(2000, {"final": lp}, plain_iters=(2, 1, 1))
+    assert r["plain_iters"] == 4 and r["plain_s"] > 0 and np.isfinite(r["plain_rmse_final"])
+    L = r["learned"]["final"]
+    assert L["iters"] <= 4 and L["learned_steps"] + L["fallbacks"] == L["iters"]
+    assert np.isfinite(L["rmse_final"])
+    if L["reached"]:
+        assert L["rmse_final"] <= r["plain_rmse_final"] + 1e-9 and 0 <= L["time_ratio"]

@@ -55,18 +55,45 @@ def test_projections_closed_form():
     assert np.all(vp2 ** 2 - 4.0 / 3.0 * vs2 ** 2 > 0)
 
 
-def test_plateau_detector():
-    from stfwi_amd.fwi.optim_lbfgsb import _Plateau, _active_keys
-    p = _Plateau(2, 0.1)
-    assert not any(p.push(v) for v in (10.0, 9.0, 5.0))
-    # mean(last 2) = (5 + 4.9)/2 >= 0.9 * mean(10, 9)?  4.95 < 8.55 -> keep going
-    assert not p.push(4.9)
-    assert not p.push(4.9)      # mean(4.9, 4.9) = 4.9 < 0.9 * mean(9, 5) = 6.3
-    assert p.push(4.9)          # mean(4.9, 4.9) >= 0.9 * mean(5, 4.9)
-    assert not _Plateau(None, 0.1).push(1.0)
-    assert _active_keys("vs_vp") == ["Vs", "Vp"]
+def test_driver_plateau_stop_stage_parsing_and_coordinates():
+    """The multistage driver's pieces on the host: the plateau stop (mean of
+    the last `window` misfits not below (1 - tol) times the mean of the
+    window before, fwi/optim_lbfgsb.py:117-134), stage parsing with the
+    reference's errors, and the scaled coordinates with their bounds."""
+    import torch
+    from stfwi_amd.fwi.optim_lbfgsb import _Checkpoints, _Coordinates, _Objective, _Stalled, _StageSpec
+    model = tuple(torch.full((2, 3), v, dtype=torch.float64) for v in (1000.0, 2000.0, 2300.0))
+    coords = _Coordinates(model, (0,), (None, None, None))
+    x = coords.vector(model)
+
+    def objective(window, losses):
+        it = iter(losses)
+        return _Objective(coords, lambda m: m, lambda m: ((m[0] * 0, m[1] * 0, m[2] * 0), next(it)),
+                          _Checkpoints(None, 1, None), window, 0.1, False)
+
+    ob = objective(2, [10.0, 9.0, 5.0, 4.9, 4.9, 4.9])
+    for _ in range(5):       # 4.95 < 0.9 * 9.5 and 4.9 < 0.9 * 7: keep going
+        ob(x)
+    with pytest.raises(_Stalled):
+        ob(x)                # 4.9 >= 0.9 * 4.95
+    ob = objective(None, [1.0] * 8)
+    for _ in range(8):
+        ob(x)
+    assert ob.losses == [1.0] * 8
+    assert _StageSpec.parse(dict(f_c=8.0, invert="vs_vp")).invert == "vs_vp"
+    assert _StageSpec.parse(dict(f_c=8.0)).n_iters == 50
     with pytest.raises(ValueError):
-        _active_keys("vp")
+        _StageSpec.parse(dict(f_c=8.0, invert="vp"))
+    with pytest.raises(ValueError):
+        _StageSpec.parse(dict(f_c=8.0, precon="other"))
+    c2 = _Coordinates(model, (0, 1), ((500.0, 4500.0), None, None))
+    x2 = c2.vector(model)
+    assert x2.numel() == 12 and float(x2.abs().max()) == 1.0
+    back = c2.model(x2)
+    assert torch.equal(back[0], model[0]) and torch.equal(back[1], model[1]) and back[2] is model[2]
+    assert torch.equal(c2.lo[:6], torch.full((6,), 0.5, dtype=torch.float64)) and torch.isinf(c2.lo[6:]).all()
+    g = c2.gradient((torch.ones(2, 3), 2 * torch.ones(2, 3), torch.ones(2, 3)))
+    assert torch.equal(g, torch.cat([torch.full((6,), 1000.0), torch.full((6,), 4000.0)]).double())
 
 
 # ---------------------------------------------------------------- GPU ----
@@ -264,7 +291,7 @@ def test_gpu_learned_driver_collects_pairs_and_keeps_descending(tmp_path):
     import torch
     from stfwi_amd.fwi import forward_jax, ricker_jax
     from stfwi_amd.fwi.optim_lbfgsb import run_multiscale_fwi
-    from stfwi_amd.learned import LearnedPreconditioner
+    from stfwi_amd.learned import LearnedPreconditioner, PairCollector
     from stfwi_amd.models import anomaly_model
     nz, nx, pad, nt, dt, dx = 40, 64, 10, 300, 1e-3, 10.0
     vs_t, vp_t, rho_t = anomaly_model(nz, nx, seed=1)
@@ -280,10 +307,39 @@ def test_gpu_learned_driver_collects_pairs_and_keeps_descending(tmp_path):
                   free_surface=True, verbose=False)
     stage = [dict(f_c=15.0, n_iters=4, invert="vs", illumination_eps=1e-3)]
     lp = LearnedPreconditioner(nz, nx, seed=0)
-    params, hist = run_multiscale_fwi(Vs_init=vs0, stages=stage, collect=lp, **common)
-    assert len(lp.pairs) == 4
-    lp.train(epochs=60, lr=3e-3, batch=2)
+    col = PairCollector(lp)
+    params, hist = run_multiscale_fwi(Vs_init=vs0, stages=stage, collect=col, **common)
+    assert len(col) == 4 and len(col.pairs["ahead"]) == 4
+    lp.train(epochs=60, lr=3e-3, batch=2, pairs=col.pairs["step"])
     params2, hist2 = run_multiscale_fwi(Vs_init=params[0], stages=stage, learned=lp, **common)
     h = hist2[0]
     assert h["n_learned"] + h["n_fallback"] == 4
     assert h["final_loss"] < hist[0]["final_loss"]
+
+
+def test_tikhonov_term_gradient_by_central_differences():
+    """The Tikhonov term of build_loss_fn (fwi/loss.py:123-128): the sum of
+    squared first differences along z and x of the inverted parameters.
+    Its gradient against central differences, exact for a quadratic; the
+    frozen parameters get none."""
+    import torch
+    from stfwi_amd.fwi.loss import _tikhonov
+    rng = np.random.default_rng(3)
+    params = [torch.as_tensor(rng.standard_normal((5, 7)), dtype=torch.float64) for _ in range(3)]
+    for invert, active in (("vs", 1), ("vs_vp", 2), ("all", 3)):
+        reg, grads = _tikhonov(params, invert)
+        ref = sum(float((np.diff(p.numpy(), axis=0) ** 2).sum() + (np.diff(p.numpy(), axis=1) ** 2).sum())
+                  for p in params[:active])
+        assert abs(reg / ref - 1) < 1e-14
+        for i, p in enumerate(params):
+            if i >= active:
+                assert not torch.any(grads[i])
+                continue
+            for (r, c) in ((0, 0), (2, 3), (4, 6), (1, 6)):
+                h = 1e-3
+                pp = [q.clone() for q in params]
+                pp[i][r, c] += h
+                fp = _tikhonov(pp, invert)[0]
+                pp[i][r, c] -= 2 * h
+                fm = _tikhonov(pp, invert)[0]
+                assert abs((fp - fm) / (2 * h) - float(grads[i][r, c])) < 1e-8
