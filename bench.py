@@ -222,42 +222,52 @@ def batched_bench(steps, warmup):
             "tiles": {fd: r[5] for fd, r in zip((4, 8), runs)}}
 
 
-def learned_vs_plain(seeds, weights, mode):
+def learned_vs_plain(seeds, weights, spec):
     """Time to the plain method's final Vs RMSE, learned-preconditioned
     against plain L-BFGS, on held-out seeded layered models
     (stfwi_amd.learned_eval; 25 moment shots, 152x500, nt=3000, FP32 'ad'
-    gradients; bands 3/5/8 Hz x 20 iterations).  Prints one JSON line."""
-    import torch
-    from stfwi_amd.learned import LearnedPreconditioner
+    gradients; bands 3/5/8 Hz x 20 iterations).  `spec` is the network's
+    target and how the driver uses it, mode:policy:maxls.  Prints one JSON
+    line; per-seed rows go to stderr as they finish."""
     from stfwi_amd import learned_eval as E
-    lp = LearnedPreconditioner.load(os.path.join(weights, f"{mode}.pt"))
+    name = spec
+    lp = E.variant(weights, spec)
     rows = []
     for seed in seeds:
-        r = E.evaluate_seed(seed, {mode: lp})
-        print(json.dumps(r), file=sys.stderr, flush=True)
+        r = E.evaluate_seed(seed, {name: lp})
+        print(json.dumps({k: v for k, v in r.items() if k != "plain_trace"}), file=sys.stderr, flush=True)
         rows.append(r)
     ok = [r for r in rows if r["plain_improves"]]
-    reached = [r for r in ok if r["learned"][mode]["reached"]]
-    ratios = [r["learned"][mode]["time_ratio"] for r in reached]
+    reached = [r for r in ok if r["learned"][name]["reached"]]
+    ratios = [r["learned"][name]["time_ratio"] for r in reached]
     plain_s = sum(r["plain_s"] for r in ok)
     # a seed whose learned run never reaches the plain RMSE counts with its
     # whole run time (a lower bound on its time to the target)
-    learned_s = sum(r["learned"][mode]["seconds"] if r["learned"][mode]["reached"] else r["learned"][mode]["run_s"]
+    learned_s = sum(r["learned"][name]["seconds"] if r["learned"][name]["reached"] else r["learned"][name]["run_s"]
                     for r in ok)
+    trained = None
+    sfile = os.path.join(weights, "seeds.txt")
+    if os.path.exists(sfile):
+        with open(sfile) as f:
+            trained = [ln.strip() for ln in f if ln.strip()]
     return {"metric": "time to the plain method's final Vs RMSE, learned vs plain (ratio)",
             "value": learned_s / plain_s if plain_s else None, "unit": "learned seconds / plain seconds",
             "higher_is_better": False, "n_gpus": 1, "dtype": "f32", "data": "synthetic (seeded layered models)",
             "config": {"workload": "25-shot 152x500 nt=3000 FWI, Vs only, bands 3/5/8 Hz x 20 L-BFGS iterations",
-                       "held_out_seeds": [int(s) for s in seeds], "target": mode, "weights": weights},
+                       "held_out_seeds": [int(s) for s in seeds], "variant": spec, "weights": weights,
+                       "training_runs": trained},
             "seeds_where_plain_lowers_rmse": len(ok), "reached": len(reached),
             "median_time_ratio_of_reached": float(np.median(ratios)) if ratios else None,
             "plain_seconds": plain_s, "learned_seconds": learned_s,
-            "learned_steps": sum(r["learned"][mode]["learned_steps"] for r in ok),
-            "fallbacks": sum(r["learned"][mode]["fallbacks"] for r in ok),
+            "plain_evaluations": sum(r["plain_evals"] for r in ok),
+            "learned_evaluations": sum(r["learned"][name]["evals"] for r in ok),
+            "learned_steps": sum(r["learned"][name]["learned_steps"] for r in ok),
+            "fallbacks": sum(r["learned"][name]["fallbacks"] for r in ok),
             "per_seed": [{"seed": r["seed"], "rmse0": round(r["rmse_initial"], 2),
                           "plain_rmse": round(r["plain_rmse_final"], 2), "plain_s": round(r["plain_s"], 2),
-                          "learned_s": r["learned"][mode]["seconds"], "learned_rmse": round(
-                              r["learned"][mode]["rmse_final"], 2)} for r in rows]}
+                          "learned_s": r["learned"][name]["seconds"], "learned_run_s": round(
+                              r["learned"][name]["run_s"], 2), "learned_rmse": round(
+                              r["learned"][name]["rmse_final"], 2)} for r in rows]}
 
 
 def main():
@@ -280,7 +290,8 @@ def main():
     ap.add_argument("--seeds", default="2000-2015", help="held-out seeds for --learned-vs-plain (a-b)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "stfwi_amd", "weights"),
                     help="directory of the trained preconditioner (<mode>.pt)")
-    ap.add_argument("--target", default="final", help="training target of the network to evaluate")
+    ap.add_argument("--variant", default="final:until_fail:1",
+                    help="network target and how the driver uses it, mode:policy:maxls (--learned-vs-plain)")
     ap.add_argument("--learned", action="store_true",
                     help="time the learned preconditioner's matrix-core convolution kernels and one "
                          "training epoch instead of the gradient path")
@@ -303,7 +314,7 @@ def main():
         return
     if args.learned_vs_plain:
         a, b = (int(v) for v in args.seeds.split("-"))
-        print(json.dumps(learned_vs_plain(range(a, b + 1), args.weights, args.target)))
+        print(json.dumps(learned_vs_plain(range(a, b + 1), args.weights, args.variant)))
         return
     if args.workload == "batched":
         print(json.dumps(batched_bench(args.steps, args.warmup)))

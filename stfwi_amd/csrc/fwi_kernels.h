@@ -454,6 +454,20 @@ __device__ __forceinline__ bool interior_z(const Geo<T> &g, int r0, int tz)
     return r0 >= top && r0 + tz + 8 <= g.nz - g.pad;
 }
 
+// The halo cells of a tile are staged by the block's threads in rounds:
+// round k gives entry e = tid + 256 k of the nh halo cells to thread tid.  A
+// wavefront (the 64 threads of one threadIdx.y) whose first entry of a round
+// is past the halo has no valid lane in it and skips the round's loads,
+// C-PML inputs and LDS stores with a scalar branch: threadIdx.y is uniform
+// across the wavefront.  Lanes past the halo in a wavefront that does have
+// valid lanes keep their selects (they load cell 0 and store into a spare
+// LDS row).
+__device__ __forceinline__ bool halo_round(int k, int nh)
+{
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    return 256 * k + TILE_X * w < nh;
+}
+
 #define CX(x) (IX || (x))
 #define CZ(x) (IZ || (x))
 #define STFWI_DISPATCH(ix, iz, body)                                          \
@@ -533,6 +547,7 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
     bool xok[KX];
 #pragma unroll
     for (int k = 0; k < KX; ++k) {
+        if (!halo_round(k, 8 * TZ)) continue;
         const int e = tid + 256 * k;
         const int zi = e >> 3, h = e & 7, xi = h < 4 ? h : h + TILE_X;
         const int r = r0 + zi, cc = c0 - 4 + xi;
@@ -590,6 +605,7 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
     }
 #pragma unroll
     for (int k = 0; k < KX; ++k) {
+        if (!halo_round(k, 8 * TZ)) continue;
         const int e = tid + 256 * k;
         {
             // lanes past the halo store into the spare row TZ
@@ -734,6 +750,7 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
     bool xok[KX];
 #pragma unroll
     for (int k = 0; k < KX; ++k) {
+        if (!halo_round(k, 8 * (TZ + 8))) continue;
         const int e = tid + 256 * k;
         const int zi = e >> 3, h = e & 7, xi = h < 4 ? h : h + TILE_X;
         const int r = r0 - 4 + zi, cc = c0 - 4 + xi;
@@ -780,6 +797,7 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
     }
 #pragma unroll
     for (int k = 0; k < KX; ++k) {
+        if (!halo_round(k, 8 * (TZ + 8))) continue;
         const int e = tid + 256 * k;
         {
             // lanes past the halo store into the spare row TZ + 8
@@ -1103,6 +1121,7 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
     bool hok[KX];
 #pragma unroll
     for (int k = 0; k < KX; ++k) {
+        if (!halo_round(k, 8 * TZ)) continue;
         const int e = tid + 256 * k;
         const int zi = e >> 3, h = e & 7, xi = h < 4 ? h : h + TILE_X;
         const int r = r0 + zi, cc = c0 - 4 + xi;
@@ -1139,6 +1158,7 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
         }
 #pragma unroll
         for (int k = 0; k < KX; ++k) {
+            if (!halo_round(k, 8 * TZ)) continue;
             const int e = tid + 256 * k;
             {
                 // lanes past the halo store into the spare row TZ
@@ -1165,6 +1185,7 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
         int ihs[KX];
 #pragma unroll
         for (int k = 0; k < KX; ++k) {
+            if (!halo_round(k, 8 * TZ)) continue;
             const int e = tid + 256 * k;
             const int zi = e >> 3, h = e & 7, xi = h < 4 ? h : h + TILE_X;
             ihs[k] = (!IX && hok[k]) ? xstrip(g, r0 + zi, c0 - 4 + xi) : -1;
@@ -1208,6 +1229,7 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
         }
 #pragma unroll
         for (int k = 0; k < KX; ++k) {
+            if (!halo_round(k, 8 * TZ)) continue;
             const int e = tid + 256 * k;
             {
                 // lanes past the halo store into the spare row TZ
@@ -1448,6 +1470,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
     bool hok[KX];
 #pragma unroll
     for (int k = 0; k < KX; ++k) {
+        if (!halo_round(k, 8 * TZ)) continue;
         const int e = tid + 256 * k;
         const int zi = e >> 3, h = e & 7, xi = h < 4 ? h : h + TILE_X;
         const int r = r0 + zi, cc = c0 - 4 + xi;
@@ -1525,6 +1548,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
         }
 #pragma unroll
         for (int k = 0; k < KX; ++k) {
+            if (!halo_round(k, 8 * TZ)) continue;
             const int e = tid + 256 * k;
             {
                 // lanes past the halo store into the spare row TZ
@@ -1552,6 +1576,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
         int ihs[KX];
 #pragma unroll
         for (int k = 0; k < KX; ++k) {
+            if (!halo_round(k, 8 * TZ)) continue;
             const int e = tid + 256 * k;
             const int zi = e >> 3, h = e & 7, xi = h < 4 ? h : h + TILE_X;
             ihs[k] = (!IX && hok[k]) ? xstrip(g, r0 + zi, c0 - 4 + xi) : -1;
@@ -1588,6 +1613,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
         }
 #pragma unroll
         for (int k = 0; k < KX; ++k) {
+            if (!halo_round(k, 8 * TZ)) continue;
             const int e = tid + 256 * k;
             {
                 // lanes past the halo store into the spare row TZ

@@ -74,5 +74,29 @@ def get_plan(**kw):
     return p
 
 
+# launch shapes: a plan runs its shots as up to 3 concurrent stream groups
+# when a group still fills the chip (about 1200 tiles of 8 x 64 cells per
+# launch), and picks its tile heights by timing them on its own shots once.
+# Both choices change no result bit (the kernels are tested bitwise across
+# groupings and tile heights); they only decide how fast a call runs.
+_GROUP_TILES = 1200
+_TUNE_CELLS = 1 << 20
+
+
+def tune_launches(p, n_shots):
+    """Stream groups and tile heights for plan p, once per plan, after its
+    model and shots are set (small problems keep the defaults)."""
+    if getattr(p, "_tuned", False):
+        return
+    p._tuned = True
+    nz_t, nx_t = p.shape
+    tiles = -(-nz_t // 8) * -(-nx_t // 64)
+    groups = max(1, min(3, (n_shots * tiles) // _GROUP_TILES))
+    if groups > 1:
+        p.set_groups(groups)
+    if n_shots * nz_t * nx_t >= _TUNE_CELLS:
+        p.autotune(10)
+
+
 def clear_plan_cache():
     _CACHE.clear()

@@ -36,7 +36,7 @@ from typing import Optional
 import numpy as np
 import torch
 
-from ._common import as_dev, as_int_array, compute_dtype, get_plan, storage_dtype, device
+from ._common import as_dev, as_int_array, compute_dtype, get_plan, storage_dtype, device, tune_launches
 from .loss import gaussian_smooth_2d
 
 
@@ -88,6 +88,7 @@ def compute_gradient(backend: str, *, Vs, Vp, rho, src_x, src_z, src_wavelet, ob
         b_img = (1.0 / as_dev(rho, storage_dtype(rho))).to(dtype)
     p.set_model(Vs, Vp, rho, b_img)
     p.set_shots(sx, sz, src_type if honour_src_type else "force", nt, rx, rz, as_dev(src_wavelet, dtype))
+    tune_launches(p, n)
     rec = p.forward(channels=(int(component),))["rec"]
     obs = torch.zeros((n, nt, p.nrec_max), dtype=dtype, device=device())
     obs[:, :, :nrec] = as_dev(observed, dtype).reshape(n, nt, nrec)

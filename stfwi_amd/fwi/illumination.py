@@ -17,7 +17,7 @@ import ctypes as C
 import torch
 
 from .. import _abi
-from ._common import as_dev, as_int_array, compute_dtype, forward_fd_order, get_plan, device
+from ._common import as_dev, as_int_array, compute_dtype, forward_fd_order, get_plan, device, tune_launches
 
 
 def shot_energy(nz, nx, dx, dz, dt, nt, f_c, pad, rec_x, rec_z, fd_order, component, Vs, Vp, rho, src_x, src_z,
@@ -32,6 +32,7 @@ def shot_energy(nz, nx, dx, dz, dt, nt, f_c, pad, rec_x, rec_z, fd_order, compon
                  store_history=False, device=device())
     p.set_model(Vs, Vp, rho)
     p.set_shots(sx, sz, src_type, int(nt), rx, as_int_array(rec_z), as_dev(src_wavelet, dtype))
+    tune_launches(p, sx.size)
     return p.forward(channels=(), illum_comp=int(component))["illum"][:sx.size]
 
 

@@ -33,11 +33,17 @@ L-BFGS-B; bounds are kept by projecting the trial points.
 Keyword arguments beyond the reference's:
   learned  an object with ``direction(grad, illum, vs) -> (nz, nx)`` tensor
            (stfwi_amd.learned.LearnedPreconditioner): its direction is tried
-           before every L-BFGS step of a Vs-only stage, and the plain step
-           is taken whenever it does not decrease the misfit (history keys
-           n_learned, n_fallback).  With ``learned.absolute`` true the
-           direction is an update in m/s, tried as the step itself;
-           otherwise it is scaled to the L-BFGS step;
+           before the L-BFGS step of a Vs-only stage, and the plain step is
+           taken whenever it does not decrease the misfit within
+           ``learned.maxls`` evaluations (default 10; history keys
+           n_learned, n_fallback).  ``learned.policy`` says when it is
+           tried: 'every' iteration (default), every iteration of a stage
+           'until_fail' (until one proposal of the stage is rejected), the
+           first iteration of each Vs-only 'stage', or only the first
+           iteration of the 'run'.  With
+           ``learned.absolute`` true the direction is an update in m/s,
+           tried as the step itself; otherwise it is scaled to the L-BFGS
+           step;
   collect  an object with ``add_step(grad, illum, vs, step)``,
            ``end_stage(vs)`` and optionally ``end_run(vs)``
            (stfwi_amd.learned.PairCollector) that receives the accepted steps
@@ -252,6 +258,7 @@ def run_multiscale_fwi(*, Vs_init, Vp_init, rho_init, src_x, src_z, rec_x, rec_z
     history = []
     t_run = time.time()
     stopped = False
+    learned_used = [False]
 
     for k, spec in enumerate(specs):
         if verbose:
@@ -289,47 +296,66 @@ def run_multiscale_fwi(*, Vs_init, Vp_init, rho_init, src_x, src_z, rec_x, rec_z
         objective = _Objective(coords, project, gradient_at, ckpt, plateau_window, plateau_rel_tol, verbose)
         vs_only = blocks == (0,)
         n_it = [0]
+        accepted = [None]
         if monitor is not None and monitor(k, 0, model[0], time.time() - t_run):
             stopped = True
             break
 
         def after_iteration(x):
             n_it[0] += 1
+            accepted[0] = x
             m = project(coords.model(x))
             ckpt.iteration(n_it[0], m, objective.losses[-1] if objective.losses else float("nan"))
             if monitor is not None and monitor(k, n_it[0], m[0], time.time() - t_run):
                 raise _MonitorStop()
 
         absolute = bool(getattr(learned, "absolute", False))
+        policy = getattr(learned, "policy", "every")
+        counts = {"proposed": 0, "learned": 0}
 
         def propose(x, g):
+            # 'every': before every L-BFGS step; 'until_fail': before every
+            # step of a stage until one proposal is rejected; 'stage': at the
+            # first iteration of each Vs-only stage; 'run': at the first
+            # iteration of the run's first Vs-only stage
+            if (policy == "stage" and (n_it[0] > 0 or counts["proposed"] > 0)) or \
+                    (policy == "run" and learned_used[0]) or \
+                    (policy == "until_fail" and counts["proposed"] > counts["learned"]):
+                return None
+            counts["proposed"] += 1
+            learned_used[0] = True
             # an update in model units becomes a step in the optimiser's
             # coordinates; a unit-free direction is scaled by the line search
             m = project(coords.model(x))
             d = learned.direction(g.reshape(grid), illum, m[0]).reshape(-1)
             return d / coords.scale[0] if absolute else d
 
-        def record_step(x_old, g_old, x_new, g_new, was_learned):
-            m = project(coords.model(x_old))
-            collect.add_step(g_old.reshape(grid), illum, m[0], (x_new - x_old).reshape(grid) * coords.scale[0])
+        def on_step(x_old, g_old, x_new, g_new, was_learned):
+            counts["learned"] += int(bool(was_learned))
+            if collect is not None:
+                m = project(coords.model(x_old))
+                collect.add_step(g_old.reshape(grid), illum, m[0], (x_new - x_old).reshape(grid) * coords.scale[0])
 
         t_stage = time.time()
-        n_learned = n_fallback = 0
         try:
             res = minimize(objective, coords.vector(model), lo=coords.lo, hi=coords.hi, maxiter=spec.n_iters, m=10,
                            maxls=20, callback=after_iteration,
                            direction_fn=propose if (learned is not None and vs_only) else None,
-                           on_step=record_step if (collect is not None and vs_only) else None,
+                           maxls_learned=int(getattr(learned, "maxls", 10)),
+                           on_step=on_step if vs_only else None,
                            learned_step="own" if (learned is not None and absolute) else "lbfgs")
             x_end, f_end, ok = res["x"], float(res["fun"]), bool(res["success"])
-            n_learned, n_fallback = res["n_learned"], res["n_fallback"]
         except _Stalled:
             x_end, ok = objective.last_x, False
             f_end = objective.losses[-1] if objective.losses else float("nan")
         except _MonitorStop:
-            x_end, ok, stopped = objective.last_x, True, True
+            x_end, ok, stopped = accepted[0], True, True
             f_end = objective.losses[-1] if objective.losses else float("nan")
         model = project(coords.model(x_end))
+        # a proposal is rejected when the accepted step of its iteration was
+        # not the learned one
+        n_learned = counts["learned"]
+        n_fallback = counts["proposed"] - n_learned
         if collect is not None and vs_only:
             collect.end_stage(model[0])
         seconds = time.time() - t_stage

@@ -210,8 +210,9 @@ def minimize(fun: Callable, x0: torch.Tensor, lo: Optional[torch.Tensor] = None,
     One iteration = one accepted step; callback(x) runs after each.  Stops
     after maxiter iterations or when the line search finds no decrease.
 
-    direction_fn(x, g) -> d (or None) proposes a search direction, e.g. a
-    learned preconditioner's.  It is tried first, scaled to the L-BFGS
+    direction_fn(x, g) -> d (or None: no proposal at this iterate) proposes
+    a search direction, e.g. a learned preconditioner's.  It is tried first,
+    scaled to the L-BFGS
     direction's max |.| (learned_step='lbfgs') or taken as the step itself
     (learned_step='own': the first trial is x + d), with projected
     backtracking of at most maxls_learned evaluations; when it is not a
@@ -287,10 +288,10 @@ def minimize(fun: Callable, x0: torch.Tensor, lo: Optional[torch.Tensor] = None,
                     nfev += ev.nfev
                     if r is not None and r[1] < f:
                         res, learned = r, True
-            if learned:
-                n_learned += 1
-            else:
-                n_fallback += 1
+                if learned:
+                    n_learned += 1
+                else:
+                    n_fallback += 1
         ev = _Eval(fun, x, d, lo, hi)
         if res is not None:
             pass

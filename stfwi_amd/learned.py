@@ -260,7 +260,8 @@ class LearnedPreconditioner:
 
     absolute = True
 
-    def __init__(self, nz, nx, channels=(16, 32, 64, 128), seed=0, device=None, vs_ref=3000.0, step_ref=100.0):
+    def __init__(self, nz, nx, channels=(16, 32, 64, 128), seed=0, device=None, vs_ref=3000.0, step_ref=100.0,
+                 policy="every", maxls=10):
         self.nz, self.nx = int(nz), int(nx)
         self.net = UNet(3, 1, channels, seed, device)
         m = self.net.multiple
@@ -268,6 +269,12 @@ class LearnedPreconditioner:
         self.device = self.net.device
         self.vs_ref = float(vs_ref)
         self.step_ref = float(step_ref)
+        # how the FWI driver uses it: when a proposal is made ('every'
+        # iteration, each 'stage' start, or the 'run' start) and how many
+        # misfit evaluations its backtracking may spend before the plain
+        # step is taken
+        self.policy = policy
+        self.maxls = int(maxls)
         self.pairs: list = []
 
     def features(self, grad, illum, vs):
@@ -297,9 +304,12 @@ class LearnedPreconditioner:
     def add_pair(self, grad, illum, vs, step):
         self.pairs.append((self.features(grad, illum, vs), self.target(step)))
 
-    def train(self, epochs=200, lr=1e-3, batch=4, seed=0, pairs=None):
+    def train(self, epochs=200, lr=1e-3, batch=4, seed=0, pairs=None, flip=False):
         """Fit the pairs (MSE over the physical cells); returns the loss
-        history (one value per epoch, mean over batches)."""
+        history (one value per epoch, mean over batches).  flip=True mirrors
+        the physical columns of every other batch (a survey with sources
+        and receivers spread evenly along the surface is nearly symmetric
+        under x -> nx - 1 - x)."""
         pairs = self.pairs if pairs is None else pairs
         if not pairs:
             return []
@@ -307,6 +317,7 @@ class LearnedPreconditioner:
         wgt = torch.zeros((1, 1, self.H, self.W), dtype=torch.float32, device=self.device)
         wgt[0, 0, :self.nz, :self.nx] = 1.0
         hist = []
+        nx = self.nx
         for _ in range(epochs):
             order = rng.permutation(len(pairs))
             tot, nb = 0.0, 0
@@ -314,6 +325,9 @@ class LearnedPreconditioner:
                 idx = order[i:i + batch]
                 x = torch.cat([pairs[j][0] for j in idx])
                 t = torch.cat([pairs[j][1] for j in idx])
+                if flip and rng.random() < 0.5:
+                    x[..., :nx] = x[..., :nx].flip(-1)
+                    t[..., :nx] = t[..., :nx].flip(-1)
                 y = self.net.forward(x)
                 loss, gy = mse(y, t, wgt.expand_as(t).contiguous())
                 self.net.adam_step(self.net.backward(gy), lr=lr)
@@ -322,15 +336,29 @@ class LearnedPreconditioner:
             hist.append(tot / nb)
         return hist
 
-    def save(self, path):
-        torch.save({"state": self.net.state_dict(), "channels": self.net.channels, "nz": self.nz, "nx": self.nx,
-                    "vs_ref": self.vs_ref, "step_ref": self.step_ref}, path)
+    def save(self, path, optimizer=True):
+        """Weights and settings, with the Adam moments unless optimizer is
+        False (with them, training can continue from the file)."""
+        net = self.net
+        d = {"state": net.state_dict(), "channels": net.channels, "nz": self.nz, "nx": self.nx,
+             "vs_ref": self.vs_ref, "step_ref": self.step_ref}
+        if optimizer:
+            d["adam"] = {f"{n}.{j}.{i}": t.detach().cpu() for n, mv in net.adam.items() for j in range(2)
+                         for i, t in enumerate(mv[j])}
+            d["adam_step"] = net.step_count
+        torch.save(d, path)
 
     @classmethod
     def load(cls, path, device=None):
         d = torch.load(path, map_location="cpu", weights_only=True)
         lp = cls(d["nz"], d["nx"], tuple(d["channels"]), device=device, vs_ref=d["vs_ref"], step_ref=d["step_ref"])
         lp.net.load_state_dict(d["state"])
+        if "adam" in d:
+            for n, mv in lp.net.adam.items():
+                for j in range(2):
+                    for i in range(len(mv[j])):
+                        mv[j][i].copy_(d["adam"][f"{n}.{j}.{i}"].to(lp.device))
+            lp.net.step_count = int(d["adam_step"])
         return lp
 
 

@@ -192,7 +192,7 @@ def test_learned_vs_plain_evaluation_runs_end_to_end():
     r = E.evaluate_seed(2000, {"final": lp}, plain_iters=(2, 1, 1))
     assert r["plain_iters"] == 4 and r["plain_s"] > 0 and np.isfinite(r["plain_rmse_final"])
     L = r["learned"]["final"]
-    assert L["iters"] <= 4 and L["learned_steps"] + L["fallbacks"] == L["iters"]
+    assert L["iters"] <= 4 and L["learned_steps"] <= L["iters"] <= L["learned_steps"] + L["fallbacks"]
     assert np.isfinite(L["rmse_final"])
     if L["reached"]:
         assert L["rmse_final"] <= r["plain_rmse_final"] + 1e-9 and 0 <= L["time_ratio"]

@@ -317,6 +317,60 @@ def test_gpu_learned_driver_collects_pairs_and_keeps_descending(tmp_path):
     assert h["final_loss"] < hist[0]["final_loss"]
 
 
+class _StubProposer:
+    """A learned-direction stand-in: steepest descent (accepted) or ascent
+    (rejected without an evaluation), proposed per `policy`."""
+    absolute = False
+
+    def __init__(self, policy, sign=-1.0):
+        self.policy, self.maxls, self.sign, self.calls = policy, 2, sign, 0
+
+    def direction(self, grad, illum, vs):
+        self.calls += 1
+        return self.sign * grad
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("policy,sign,want", [
+    ("every", -1.0, (3, 3)), ("stage", -1.0, (1, 1)), ("run", -1.0, (1, 0)),
+    ("until_fail", 1.0, (1, 1)), ("until_fail", -1.0, None)])
+def test_gpu_learned_proposal_policies(policy, sign, want):
+    """When the driver asks for a learned update: at every iteration, at the
+    first iteration of each stage, once per run, or at every iteration
+    until one proposal of the stage is rejected.  want = proposals in the
+    first and second stage (3 iterations each); for 'until_fail' with a
+    descent proposal, the proposals of a stage are its accepted learned
+    steps plus at most the one rejection that ends them."""
+    import torch
+    from stfwi_amd.fwi import forward_jax, ricker_jax
+    from stfwi_amd.fwi.optim_lbfgsb import run_multiscale_fwi
+    from stfwi_amd.models import anomaly_model
+    nz, nx, pad, nt, dt, dx = 40, 64, 10, 300, 1e-3, 10.0
+    vs_t, vp_t, rho_t = anomaly_model(nz, nx, seed=1)
+    sx, sz = np.array([10, 32, 54]), np.array([2, 2, 2])
+    rx = np.arange(0, nx, 2)
+    rz = np.full_like(rx, 1)
+    wav = ricker_jax(torch.arange(nt, dtype=torch.float64) * dt, 15.0, 1.5 / 15.0)
+    obs = np.stack([forward_jax(vs_t, vp_t, rho_t, int(a), int(b), rx, rz, nx, nz, dx, dx, dt, nt, 15.0, pad, wav,
+                                fd_order=4, free_surface=True)[2].cpu().numpy() for a, b in zip(sx, sz)])
+    stages = [dict(f_c=15.0, n_iters=3, invert="vs", illumination_eps=1e-3)] * 2
+    prop = _StubProposer(policy, sign)
+    _, hist = run_multiscale_fwi(Vs_init=np.full_like(vs_t, 1730.0), Vp_init=vp_t, rho_init=rho_t, src_x=sx,
+                                 src_z=sz, rec_x=rx, rec_z=rz, nx_dom=nx, nz_dom=nz, dx=dx, dz=dx, dt=dt, nt=nt,
+                                 pad=pad, observed=obs, stages=stages, component=2, backend="ad", fd_order=4,
+                                 free_surface=True, verbose=False, learned=prop)
+    got = tuple(h["n_learned"] + h["n_fallback"] for h in hist)
+    if want is None:
+        assert all(h["n_fallback"] <= 1 and 1 <= h["n_learned"] + h["n_fallback"] <= 3 for h in hist), hist
+        assert all(h["n_fallback"] == 1 or h["n_learned"] == 3 for h in hist)
+        want = got
+    assert got == want, (got, want)
+    assert prop.calls == sum(want)
+    if sign > 0:
+        assert all(h["n_learned"] == 0 for h in hist)
+    assert all(h["n_iters_done"] == 3 for h in hist)
+
+
 def test_tikhonov_term_gradient_by_central_differences():
     """The Tikhonov term of build_loss_fn (fwi/loss.py:123-128): the sum of
     squared first differences along z and x of the inverted parameters.
