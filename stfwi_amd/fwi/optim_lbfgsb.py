@@ -327,7 +327,7 @@ def run_multiscale_fwi(*, Vs_init, Vp_init, rho_init, src_x, src_z, rec_x, rec_z
             # an update in model units becomes a step in the optimiser's
             # coordinates; a unit-free direction is scaled by the line search
             m = project(coords.model(x))
-            d = learned.direction(g.reshape(grid), illum, m[0]).reshape(-1)
+            d = learned.direction(g.reshape(grid), illum, m[0]).reshape(-1) * float(getattr(learned, "scale", 1.0))
             return d / coords.scale[0] if absolute else d
 
         def on_step(x_old, g_old, x_new, g_new, was_learned):

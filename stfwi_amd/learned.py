@@ -275,6 +275,8 @@ class LearnedPreconditioner:
         # step is taken
         self.policy = policy
         self.maxls = int(maxls)
+        # factor on the predicted update before it is tried
+        self.scale = 1.0
         self.pairs: list = []
 
     def features(self, grad, illum, vs):

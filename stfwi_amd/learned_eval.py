@@ -194,14 +194,17 @@ def _seeds(text):
 
 
 def variant(weights_dir, spec):
-    """'mode:policy:maxls' -> the network trained for target `mode` under
-    `weights_dir`, set to be proposed per `policy` with at most `maxls`
-    evaluations of backtracking."""
+    """'mode:policy:maxls[:scale][@dir]' -> the network trained for target
+    `mode` under `weights_dir` (or `dir`), set to be proposed per `policy`
+    with at most `maxls` evaluations of backtracking, its update multiplied
+    by `scale` (default 1)."""
     import os
     from .learned import LearnedPreconditioner
-    mode, policy, maxls = (spec.split(":") + ["every", "10"])[:3]
+    if "@" in spec:
+        spec, weights_dir = spec.split("@", 1)
+    mode, policy, maxls, scale = (spec.split(":") + ["every", "10", "1"][len(spec.split(":")) - 1:])[:4]
     lp = LearnedPreconditioner.load(os.path.join(weights_dir, f"{mode}.pt"))
-    lp.policy, lp.maxls = policy, int(maxls)
+    lp.policy, lp.maxls, lp.scale = policy, int(maxls), float(scale)
     return lp
 
 

@@ -196,3 +196,27 @@ def test_learned_vs_plain_evaluation_runs_end_to_end():
     assert np.isfinite(L["rmse_final"])
     if L["reached"]:
         assert L["rmse_final"] <= r["plain_rmse_final"] + 1e-9 and 0 <= L["time_ratio"]
+
+
+def test_trained_network_reaches_plain_rmse_with_fewer_evaluations():
+    """What the shipped network gains over the method without it, on one
+    layered model outside its training seeds (seed 3001, 25 shots, 152x500,
+    nt = 3000): the plain run (bands 3/5/8 Hz x 20 L-BFGS iterations) ends
+    at some Vs RMSE; the run with the network's update tried at the start of
+    each band (final:stage:2) reaches that RMSE with fewer misfit-and-
+    gradient evaluations.  Both runs are deterministic.  This is one model:
+    over the held-out seeds 2000-2015 the gain is not reliable, which
+    `bench.py --learned-vs-plain` measures."""
+    import os
+    from stfwi_amd import learned_eval as E
+    wdir = os.path.join(os.path.dirname(E.__file__), "weights")
+    assert os.path.exists(os.path.join(wdir, "final.pt")), "the trained network ships in stfwi_amd/weights"
+    lp = E.variant(wdir, "final:stage:2")
+    r = E.evaluate_seed(3001, {"v": lp})
+    L = r["learned"]["v"]
+    print(f"plain: RMSE {r['rmse_initial']:.2f} -> {r['plain_rmse_final']:.2f} m/s, {r['plain_evals']} evaluations, "
+          f"{r['plain_s']:.1f} s; learned: reached {L['reached']} after {L['evals']} evaluations, "
+          f"{L['run_s']:.1f} s, {L['learned_steps']} learned steps")
+    assert r["plain_improves"] and L["reached"]
+    assert L["learned_steps"] >= 1
+    assert L["evals"] < r["plain_evals"]
