@@ -79,6 +79,9 @@ struct stfwi_plan {
     int lean = 1;
     hipStream_t gstream[STFWI_MAX_GROUPS] = {};
     hipEvent_t ev_fork = nullptr, ev_join[STFWI_MAX_GROUPS] = {};
+    // group k + 1 starts its time loop after group k's first kernel, so
+    // that concurrent groups run different half-steps (stagger_in)
+    hipEvent_t ev_stag[STFWI_MAX_GROUPS] = {};
     // graph replay of the time loops: the forward and the adjoint are each
     // captured once per launch configuration (key) and replayed; their
     // inputs and outputs go through plan-owned buffers so that the captured
@@ -176,6 +179,7 @@ extern "C" int stfwi_plan_destroy(stfwi_plan *plan)
     for (int k = 0; k < STFWI_MAX_GROUPS; ++k) {
         if (plan->gstream[k]) (void)hipStreamDestroy(plan->gstream[k]);
         if (plan->ev_join[k]) (void)hipEventDestroy(plan->ev_join[k]);
+        if (plan->ev_stag[k]) (void)hipEventDestroy(plan->ev_stag[k]);
     }
     if (plan->ev_fork) (void)hipEventDestroy(plan->ev_fork);
     for (int k = 0; k < 2; ++k)
@@ -346,6 +350,7 @@ extern "C" int stfwi_plan_create(const stfwi_config *cfg, stfwi_plan **out)
     for (int k = 0; k < STFWI_MAX_GROUPS; ++k) {
         HIP_TRY(hipStreamCreateWithFlags(&p->gstream[k], hipStreamNonBlocking));
         HIP_TRY(hipEventCreateWithFlags(&p->ev_join[k], hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&p->ev_stag[k], hipEventDisableTiming));
     }
     HIP_TRY(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
     HIP_TRY(hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking));
@@ -587,6 +592,22 @@ static hipError_t fork_groups(stfwi_plan *p, int G, hipStream_t st, hipStream_t 
     return e;
 }
 
+// Staggered start of the shot groups' time loops: group k waits for the
+// first kernel of group k - 1 (stagger_in before group k's first launch,
+// stagger_out after group k's first kernel).  The groups then run offset by
+// a half-step each, so that concurrent launches are of different kernels
+// and fill each other's ramps and tails; the order of every group's own
+// launches is unchanged, so results are the same bit for bit.
+static hipError_t stagger_in(stfwi_plan *p, int k, hipStream_t *gs)
+{
+    return k > 0 ? hipStreamWaitEvent(gs[k], p->ev_stag[k - 1], 0) : hipSuccess;
+}
+
+static hipError_t stagger_out(stfwi_plan *p, int k, int G, hipStream_t *gs)
+{
+    return k + 1 < G ? hipEventRecord(p->ev_stag[k], gs[k]) : hipSuccess;
+}
+
 static hipError_t join_groups(stfwi_plan *p, int G, hipStream_t st)
 {
     if (G == 1) return hipSuccess;
@@ -717,7 +738,9 @@ static int forward_t(stfwi_plan *p, void *rec, int rec_mask, void *illum, int il
                     b.hist2_mode = 0;
                 }
             }
+            if (it == 0) HIP_TRY(stagger_in(p, k, gs));
             launch_fwd<T, FD>(0, p->tz[0], b, rblk, g0[k + 1] - g0[k], gs[k], it);
+            if (it == 0) HIP_TRY(stagger_out(p, k, G, gs));
             launch_fwd<T, FD>(1, p->tz[1], b, rblk, g0[k + 1] - g0[k], gs[k], it);
         }
     // exx/ezz receivers of the last step
@@ -890,7 +913,10 @@ static int adjoint_t(stfwi_plan *p, const void *rec_adj, void *src_adj, void *sn
                 if (first)
                     hipLaunchKernelGGL(k_adj_inject<T>, dim3((p->max_nuc + 63) / 64, 1, ns), dim3(64), 0, gs[k], a,
                                        it);
+                const bool start = seg == nseg - 1 && it == t_hi - 1;
+                if (start) HIP_TRY(stagger_in(p, k, gs));
                 launch_adj<T, FD>(2, p->tz[2], a, ns, gs[k], it);
+                if (start) HIP_TRY(stagger_out(p, k, G, gs));
                 launch_adj<T, FD>(3, p->tz[3], a, ns, gs[k], it);
             }
         }
