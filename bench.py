@@ -152,7 +152,52 @@ def learned_bench(reps=50):
     torch.cuda.synchronize()
     out["train_epoch_ms"] = round((time.perf_counter() - t0) / 5 * 1e3, 2)
     out["layer"] = f"batch {B}, {ci}->{co} channels, 3x3, {h}x{w}, BF16 operands, FP32 accumulation"
+    out["unet_layers"] = unet_layer_table(lp)
     return out
+
+
+def unet_layer_table(lp, batch=8, reps=20):
+    """Every convolution of the preconditioner's U-Net at its training shape
+    (batch 8 at its level's resolution of the padded 152x504 grid): time per
+    form (forward, data gradient, weight gradient) with HIP events, the
+    matrix-core fraction (2 B H W Cin Cout k^2 flop against 2.5 PFLOP/s
+    dense BF16) and the memory fraction (FP32 input and output activations
+    plus weights, once each, against 8 TB/s)."""
+    import torch
+    from stfwi_amd import learned as L
+    net = lp.net
+    dev = net.device
+    g = torch.Generator().manual_seed(1)
+    rows = []
+    level = {}
+    for l, c in enumerate(net.channels):
+        level[f"enc{l}a"] = level[f"enc{l}b"] = level[f"dec{l}a"] = level[f"dec{l}b"] = l
+    level["out"] = 0
+    for name, cin, cout, k in net.specs:
+        sc = 1 << level[name]
+        h, w = lp.H // sc, lp.W // sc
+        x = torch.randn((batch, cin, h, w), generator=g).to(dev)
+        gy = torch.randn((batch, cout, h, w), generator=g).to(dev)
+        wt, bias = net.params[name]
+        flops = 2.0 * batch * h * w * cin * cout * k * k
+        nbytes = 4.0 * (batch * h * w * (cin + cout) + wt.numel())
+        row = {"layer": name, "shape": f"{cin}->{cout} k{k} {h}x{w}"}
+        for form, f in (("fwd", lambda: L.conv(x, wt, bias, relu=name != "out")),
+                        ("dgrad", lambda: L.conv_dgrad(gy, wt)),
+                        ("wgrad", lambda: L.conv_wgrad(x, gy, wt.shape))):
+            f()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                f()
+            e1.record()
+            torch.cuda.synchronize()
+            us = e0.elapsed_time(e1) * 1e3 / reps
+            row[form] = {"us": round(us, 2), "mfma_frac": round(flops / (us * 1e-6) / 2.5e15, 4),
+                         "hbm_frac": round(nbytes / (us * 1e-6) / HBM_PEAK, 3)}
+        rows.append(row)
+    return rows
 
 
 def batched_bench(steps, warmup):

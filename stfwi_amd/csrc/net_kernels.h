@@ -106,7 +106,8 @@ __global__ __launch_bounds__(256) void k_conv_mfma(const float *__restrict__ in,
     }
 }
 
-// Tiled form of the same convolution for cin (of this call) <= 64.  A block
+// Tiled form of the same convolution for one pass over up to 64 input
+// channels (the host runs wider inputs as several passes).  A block
 // computes CONV_TROWS image rows x 64 columns of one image for a slice of up
 // to 32 output channels; wave w owns row w of the tile (4 MFMA column tiles
 // of 16 pixels).  Staged once per block in LDS, as BF16:
@@ -129,11 +130,12 @@ __host__ __device__ inline void conv_strides(int cinp, int Kp, int &ps, int &wps
 }
 
 // The weights in the tiled kernel's layout, BF16: wpk[ct][m][wps] holds
-// output channel ct * 32 + m at k = tap * cinp + ci (zero where no weight).
-// TR: the data-gradient weights (stored [cin][cout][ks][ks] of this call,
-// taps flipped).  One thread per destination element.
+// output channel ct * 32 + m at k = tap * cinp + ci (zero where no weight),
+// for the input channels [ci0, ci0 + cin) of the cin_all the weights hold.
+// TR: the data-gradient weights (stored [cin_all][cout][ks][ks] of this
+// call, taps flipped).  One thread per destination element.
 template <int KS, bool TR>
-__global__ void k_conv_pack(const float *__restrict__ wt, int cin, int cout, int cinp, int wps,
+__global__ void k_conv_pack(const float *__restrict__ wt, int ci0, int cin, int cin_all, int cout, int cinp, int wps,
                             __bf16 *__restrict__ wpk, long long n)
 {
     constexpr int K2 = KS * KS;
@@ -143,15 +145,21 @@ __global__ void k_conv_pack(const float *__restrict__ wt, int cin, int cout, int
         const int co = (int)row, tap = k / cinp, ci = k - tap * cinp;
         float v = 0.f;
         if (co < cout && tap < K2 && ci < cin)
-            v = TR ? wt[((long long)ci * cout + co) * K2 + (K2 - 1 - tap)] : wt[((long long)co * cin + ci) * K2 + tap];
+            v = TR ? wt[((long long)(ci0 + ci) * cout + co) * K2 + (K2 - 1 - tap)]
+                   : wt[((long long)co * cin_all + ci0 + ci) * K2 + tap];
         wpk[e] = (__bf16)v;
     }
 }
 
+// A convolution with more input channels runs as passes over chunks of
+// them: `in` points at the chunk's first channel, bstride is the input's
+// elements per image, and accum = 1 adds the pass to the partial sums in
+// `out` (the last pass alone gets the bias and the ReLU).
 template <int KS>
 __global__ __launch_bounds__(256) void k_conv_tile(const float *__restrict__ in, int cin, int H, int W,
                                                    const __bf16 *__restrict__ wpk, const float *__restrict__ bias,
-                                                   int cout, float *__restrict__ out, int relu, int cinp, int Kp)
+                                                   int cout, float *__restrict__ out, int relu, int cinp, int Kp,
+                                                   long long bstride, int accum)
 {
     constexpr int K2 = KS * KS, R = KS / 2, PR = CONV_TROWS + 2 * R, PC = 64 + 2 * R;
     extern __shared__ __attribute__((aligned(16))) unsigned char conv_smem[];
@@ -165,7 +173,7 @@ __global__ __launch_bounds__(256) void k_conv_tile(const float *__restrict__ in,
     const int nct = (cout + 31) / 32;
     const int b = blockIdx.z / nct, ct = blockIdx.z - b * nct;
     const int P = H * W;
-    const float *inb = in + (long long)b * cin * P;
+    const float *inb = in + (long long)b * bstride;
     // input patch: item = (channel group, pixel), pixels fastest; four
     // items' loads are in flight before their conversion and LDS store
     const int npix = PR * PC, ngrp = cinp / 8, nit = npix * ngrp;
@@ -249,12 +257,12 @@ __global__ __launch_bounds__(256) void k_conv_tile(const float *__restrict__ in,
         for (int i = 0; i < 4; ++i) {
             const int c0 = ct * 32 + 4 * grp + i, c1 = c0 + 16;
             if (c0 < cout) {
-                float v = acc[0][nt][i] + (bias ? bias[c0] : 0.f);
+                float v = acc[0][nt][i] + (accum ? ob[(long long)c0 * P] : 0.f) + (bias ? bias[c0] : 0.f);
                 if (relu) v = v > 0.f ? v : 0.f;
                 ob[(long long)c0 * P] = v;
             }
             if (mt_n > 1 && c1 < cout) {
-                float v = acc[1][nt][i] + (bias ? bias[c1] : 0.f);
+                float v = acc[1][nt][i] + (accum ? ob[(long long)c1 * P] : 0.f) + (bias ? bias[c1] : 0.f);
                 if (relu) v = v > 0.f ? v : 0.f;
                 ob[(long long)c1 * P] = v;
             }

@@ -1553,20 +1553,34 @@ extern "C" int stfwi_set_graphs(stfwi_plan *p, int enable)
 // learned preconditioner: BF16-MFMA convolutions and FP32 layer kernels
 // ---------------------------------------------------------------------------
 
-// the LDS-tiled convolution (input channels of the call <= 64): the weights
-// are packed once per call into a stream-ordered pool allocation, then every
-// block copies its slice.  Returns false when the shape needs the untiled
-// kernel.
+// the LDS-tiled convolution: the input channels of the call run in passes
+// of at most `chunk` channels (the first pass writes the output, the others
+// add to it, the last adds the bias and applies the ReLU).  Each pass packs
+// its slice of the weights into a stream-ordered pool allocation, and every
+// block copies its part.  Passes of 32 channels leave room in LDS for three
+// blocks per CU; a grid of fewer than 256 blocks (the coarse U-Net levels)
+// takes 64-channel passes instead, half as many launches of a grid that
+// cannot fill the chip anyway (measured faster on each layer of the 4-level
+// U-Net).  Returns false when an allocation fails (the caller then runs the
+// untiled kernel).
 template <int KS, bool TR>
 static bool conv_tile(const float *in, int batch, int cin, int h, int w, const float *wt, const float *bias, int cout,
                       float *out, int relu, hipStream_t st)
 {
-    if (cin > 64) return false;
-    const int cinp = (cin + 7) / 8 * 8, Kp = (KS * KS * cinp + 31) / 32 * 32;
-    int ps, wps;
-    conv_strides(cinp, Kp, ps, wps);
     const int PR = CONV_TROWS + KS - 1, PC = 64 + KS - 1;
-    const size_t lds = (size_t)PR * PC * ps * 2 + (size_t)32 * wps * 2 + (size_t)(Kp / 8) * 4;
+    const long long P = (long long)h * w;
+    const int nct = (cout + 31) / 32;
+    const long long nblk = (long long)((w + 63) / 64) * ((h + CONV_TROWS - 1) / CONV_TROWS) * batch * nct;
+    const int chunk = nblk < 256 ? 64 : 32;
+    auto shape = [&](int nci, int &cinp, int &Kp, int &ps, int &wps, size_t &lds) {
+        cinp = (nci + 7) / 8 * 8;
+        Kp = (KS * KS * cinp + 31) / 32 * 32;
+        conv_strides(cinp, Kp, ps, wps);
+        lds = (size_t)PR * PC * ps * 2 + (size_t)32 * wps * 2 + (size_t)(Kp / 8) * 4;
+    };
+    int cinp, Kp, ps, wps;
+    size_t lds;
+    shape(std::min(cin, chunk), cinp, Kp, ps, wps, lds);   // the largest pass
     static bool raised = false;   // the 64 KiB default limit raised once per kernel
     if (lds > 65536 && !raised) {
         if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_conv_tile<KS>),
@@ -1574,15 +1588,20 @@ static bool conv_tile(const float *in, int batch, int cin, int h, int w, const f
             return false;
         raised = true;
     }
-    const int nct = (cout + 31) / 32;
-    const long long n = (long long)nct * 32 * wps;
     void *wpk = nullptr;
-    if (hipMallocAsync(&wpk, (size_t)n * 2, st) != hipSuccess) return false;
-    hipLaunchKernelGGL((k_conv_pack<KS, TR>), dim3(grid1(n)), dim3(256), 0, st, wt, cin, cout, cinp, wps,
-                       (__bf16 *)wpk, n);
+    if (hipMallocAsync(&wpk, (size_t)nct * 32 * wps * 2, st) != hipSuccess) return false;
     dim3 grid((unsigned)((w + 63) / 64), (unsigned)((h + CONV_TROWS - 1) / CONV_TROWS), (unsigned)(batch * nct));
-    hipLaunchKernelGGL((k_conv_tile<KS>), grid, dim3(256), lds, st, in, cin, h, w, (const __bf16 *)wpk, bias, cout,
-                       out, relu, cinp, Kp);
+    for (int ci0 = 0; ci0 < cin; ci0 += chunk) {
+        const int nci = std::min(chunk, cin - ci0);
+        const bool last = ci0 + nci == cin;
+        shape(nci, cinp, Kp, ps, wps, lds);
+        const long long n = (long long)nct * 32 * wps;
+        hipLaunchKernelGGL((k_conv_pack<KS, TR>), dim3(grid1(n)), dim3(256), 0, st, wt, ci0, nci, cin, cout, cinp,
+                           wps, (__bf16 *)wpk, n);
+        hipLaunchKernelGGL((k_conv_tile<KS>), grid, dim3(256), lds, st, in + ci0 * P, nci, h, w,
+                           (const __bf16 *)wpk, last ? bias : nullptr, cout, out, last ? relu : 0, cinp, Kp,
+                           (long long)cin * P, ci0 > 0 ? 1 : 0);
+    }
     (void)hipFreeAsync(wpk, st);
     return true;
 }

@@ -20,10 +20,14 @@ def _rel(a, b):
 
 
 @pytest.mark.parametrize("k,B,cin,cout,h,w", [(3, 2, 5, 19, 13, 21), (3, 1, 16, 32, 24, 40), (1, 2, 16, 1, 10, 14),
-                                              (3, 1, 48, 16, 8, 6), (3, 2, 70, 40, 9, 70), (1, 1, 80, 33, 5, 130)])
+                                              (3, 1, 48, 16, 8, 6), (3, 2, 70, 40, 9, 70), (1, 1, 80, 33, 5, 130),
+                                              (3, 2, 192, 64, 6, 40), (3, 1, 128, 130, 5, 70)])
 def test_conv_forward_and_gradients_match_cpu(k, B, cin, cout, h, w):
     """Each kernel against the same convolution evaluated in float64 on the
-    BF16-rounded operands it consumes: only the FP32 accumulation differs."""
+    BF16-rounded operands it consumes: only the FP32 accumulation differs.
+    Inputs wider than 32 channels run as several passes that add into the
+    output (48, 70, 80, 128, 192 channels; 33, 40, 64, 130 in the data
+    gradient)."""
     from stfwi_amd import learned as L
     g = torch.Generator().manual_seed(k * 100 + cin)
     x = torch.randn((B, cin, h, w), generator=g)
