@@ -86,7 +86,7 @@ _TUNE_CELLS = 1 << 20
 def tune_launches(p, n_shots):
     """Stream groups and tile heights for plan p, once per plan, after its
     model and shots are set (small problems keep the defaults)."""
-    if getattr(p, "_tuned", False):
+    if p._tuned:
         return
     p._tuned = True
     nz_t, nx_t = p.shape

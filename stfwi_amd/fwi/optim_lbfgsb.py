@@ -235,6 +235,8 @@ def run_multiscale_fwi(*, Vs_init, Vp_init, rho_init, src_x, src_z, rec_x, rec_z
                        monitor=None, honour_src_type: bool = False):
     if not stages:
         raise ValueError("`stages` must be a non-empty list")
+    if learned is not None and getattr(learned, "policy", "every") not in ("every", "until_fail", "stage", "run"):
+        raise ValueError(f"learned.policy={learned.policy!r}; use 'every', 'until_fail', 'stage' or 'run'")
     specs = [_StageSpec.parse(s) for s in stages]
     from scipy.signal import butter
 

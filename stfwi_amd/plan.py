@@ -103,6 +103,7 @@ class Plan:
         self._keep = {}
         self.n_shots = 0
         self.nt = None
+        self._tuned = False             # launch shape chosen (fwi._common.tune_launches)
 
     def __del__(self):
         h = getattr(self, "_h", None)

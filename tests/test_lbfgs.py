@@ -317,6 +317,18 @@ def test_gpu_learned_driver_collects_pairs_and_keeps_descending(tmp_path):
     assert h["final_loss"] < hist[0]["final_loss"]
 
 
+def test_unknown_learned_policy_is_refused():
+    """run_multiscale_fwi checks learned.policy before any device work."""
+    from stfwi_amd.fwi.optim_lbfgsb import run_multiscale_fwi
+
+    class Bad:
+        policy = "sometimes"
+    with pytest.raises(ValueError, match="policy"):
+        run_multiscale_fwi(Vs_init=None, Vp_init=None, rho_init=None, src_x=None, src_z=None, rec_x=None, rec_z=None,
+                           nx_dom=4, nz_dom=4, dx=1.0, dz=1.0, dt=1e-3, nt=4, pad=2, observed=None,
+                           stages=[dict(f_c=8.0)], learned=Bad())
+
+
 class _StubProposer:
     """A learned-direction stand-in: steepest descent (accepted) or ascent
     (rejected without an evaluation), proposed per `policy`."""
