@@ -10,7 +10,10 @@
 //     (bottom pad rows first, then the top pad rows when there is no free
 //     surface).  Outside the strips the reference's recursion is an exact
 //     no-op (b = 1, a = 0, kappa = 1, psi stays 0).
-//   * histories keep the physical cells only: [shot][t][field][nz_dom*nx_dom].
+//   * histories keep the physical cells only: the strain history as
+//     [frame][field][nz_dom][hpitch] with physical column cp at cp + hcoff
+//     (hcoff = pad mod 32, so that the 64 columns of a tile start on a
+//     128-byte line), the velocity history as [frame][field][nz_dom*nx_dom].
 //
 // Every derivative keeps the reference's per-term output ranges
 // (fwi/forward.py:256-441): term 1 covers its own slice, term k >= 2 the
@@ -59,6 +62,8 @@ struct Geo {
     i64 fstride;             // elements per field buffer
     i64 origin;              // offset of cell (0,0) in a field buffer
     i64 np;                  // physical cells
+    int hpitch, hcoff;       // strain history rows: physical column cp at cp + hcoff of hpitch
+    i64 hnp;                 // strain history elements per field (nzd * hpitch)
     T dt, dx, dz, dtdx, dtdz, hdtdx, hdtdz, dzdx, fsrc;
 };
 
@@ -387,7 +392,7 @@ struct FwdArgs {
     T *rec;                          // [shot][4][nt_max][nrec_max]
     int nrec_max, rec_mask;
     const i64 *rec_cell;             // cell offset r*ld + c of every receiver
-    T *hist;                         // [frame][3][np] strains, shot s from frame hoff
+    T *hist;                         // [frame][3][hnp] strains (rows of hpitch), shot s from frame hoff
     T *hist2;                        // [frame][2][np]: 1 = dU/dB,dV/dB  2 = U,V
     int hist_nt, hist_t0;            // frames per shot with checkpoints; step of frame 0
     int hist2_mode;
@@ -877,11 +882,12 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
             if (!IZ && a.dxu0 && r == 0) a.dxu0[((i64)s * a.nt_max + it) * nx + c] = dxu;
             const int rp = r - g.z0, cp = c - g.pad;
             if (a.hist && CZ(rp >= 0 && rp < g.nzd) && CX(cp >= 0 && cp < g.nxd)) {
-                T *hb = a.hist + ((i64)sh.hoff + (it - a.hist_t0)) * 3 * g.np;
+                T *hb = a.hist + ((i64)sh.hoff + (it - a.hist_t0)) * 3 * g.hnp;
+                const int oh = rp * g.hpitch + cp + g.hcoff, nph = (int)g.hnp;
+                stst(hb, oh, exn);
+                stst(hb, oh + nph, ezn);
+                stst(hb, oh + 2 * nph, esn);
                 const int o = rp * g.nxd + cp, npi = (int)g.np;
-                stst(hb, o, exn);
-                stst(hb, o + npi, ezn);
-                stst(hb, o + 2 * npi, esn);
                 if (a.hist2_mode == 2) {
                     T *h = a.hist2 + ((i64)sh.hoff + (it - a.hist_t0)) * 2 * g.np;
                     stst(h, o, sU[zc][xc]);
@@ -935,7 +941,7 @@ struct AdjArgs {
     int nuc_max;
     T *src_adj;                        // [shot][nt_max]
     int imaging, gparams, nacc;
-    const T *hist, *hist2;             // [frame][3 | 2][np], shot s from frame hoff
+    const T *hist, *hist2;             // [frame][3][hnp] / [frame][2][np], shot s from frame hoff
     int hist_nt, hist_t0;              // frames per shot with checkpoints; step of frame 0
     const T *dxu0;
     T *gacc;                           // [shot][nacc][np]
@@ -1649,11 +1655,11 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
             const int rp = r - g.z0, cp = c - g.pad;
             const bool pok = rp >= 0 && rp < g.nzd && cp >= 0 && cp < g.nxd;
             const bool hok = hframe >= 0 && pok;
-            const T *hfr = a.hist + (hh + (hframe >= a.hist_t0 ? hframe : a.hist_t0)) * 3 * g.np;
-            const int o = hok ? rp * g.nxd + cp : 0, npi = (int)g.np;
+            const T *hfr = a.hist + (hh + (hframe >= a.hist_t0 ? hframe : a.hist_t0)) * 3 * g.hnp;
+            const int o = hok ? rp * g.hpitch + cp + g.hcoff : 0, nph = (int)g.hnp, npi = (int)g.np;
             h0[rr] = ldgg(hfr, o);
-            h1[rr] = ldgg(hfr, o + npi);
-            h2v[rr] = ldgg(hfr, o + 2 * npi);
+            h1[rr] = ldgg(hfr, o + nph);
+            h2v[rr] = ldgg(hfr, o + 2 * nph);
             // first two gradient accumulators of the cell
             const T *accb = a.gacc + (i64)s * a.nacc * g.np;
             const int oa = pok ? rp * g.nxd + cp : 0;
@@ -1858,10 +1864,10 @@ __global__ void k_hist_from_state(Geo<T> g, const T *state, i64 sstride, T *hist
         const int rp = (int)(i / g.nxd), cp = (int)(i - (i64)rp * g.nxd);
         const i64 cell = g.origin + (i64)(rp + g.z0) * g.ld + cp + g.pad;
         const T *st = state + s * sstride;
-        T *h = hist + ((i64)hoff + frame) * 3 * np + i;
+        T *h = hist + ((i64)hoff + frame) * 3 * g.hnp + (i64)rp * g.hpitch + cp + g.hcoff;
         h[0] = st[2 * g.fstride + cell];
-        h[np] = st[3 * g.fstride + cell];
-        h[2 * np] = st[4 * g.fstride + cell];
+        h[g.hnp] = st[3 * g.fstride + cell];
+        h[2 * g.hnp] = st[4 * g.fstride + cell];
     }
 }
 

@@ -49,6 +49,11 @@ struct stfwi_plan {
     int esz = 4;
     int nz = 0, nx = 0, ld = 0, z0 = 0, zrows = 0, nacc = 1;
     i64 fstride = 0, origin = 0, np = 0;
+    // strain history rows: physical column cp at cp + hcoff of a row of
+    // hpitch elements, so that a tile's 64 columns start on a 128-byte line
+    // (hcoff = pad mod 32); hnp = nz_dom * hpitch elements per field
+    int hpitch = 0, hcoff = 0;
+    i64 hnp = 0;
     i64 psix_len = 0, psiz_len = 0;
     // device buffers
     void *mod = nullptr;      // L2M, M, B, Bimg fields (4 x fstride) followed by Lr0 (nx)
@@ -155,6 +160,7 @@ static Geo<T> make_geo(const stfwi_plan *p)
     g.z0 = p->z0; g.pad = c.pad; g.nzd = c.nz_dom; g.nxd = c.nx_dom;
     g.fs = c.free_surface; g.zrows = p->zrows;
     g.fstride = p->fstride; g.origin = p->origin; g.np = p->np;
+    g.hpitch = p->hpitch; g.hcoff = p->hcoff; g.hnp = p->hnp;
     g.dt = (T)c.dt; g.dx = (T)c.dx; g.dz = (T)c.dz;
     g.dtdx = (T)(c.dt / c.dx); g.dtdz = (T)(c.dt / c.dz);
     g.hdtdx = (T)(0.5 * c.dt / c.dx); g.hdtdz = (T)(0.5 * c.dt / c.dz);
@@ -225,6 +231,9 @@ static int plan_alloc(stfwi_plan *p)
     p->fstride = (i64)(p->nz + 8) * p->ld;
     p->origin = (i64)4 * p->ld;
     p->np = (i64)c.nz_dom * c.nx_dom;
+    p->hcoff = c.pad % 32;
+    p->hpitch = (c.nx_dom + p->hcoff + 31) / 32 * 32;
+    p->hnp = (i64)c.nz_dom * p->hpitch;
     p->psix_len = (i64)p->nz * 2 * c.pad;
     p->psiz_len = (i64)p->zrows * p->nx;
     p->nacc = c.grad_params == 3 ? (c.imaging == 1 ? 3 : 4) : c.grad_params;
@@ -246,7 +255,7 @@ static int plan_alloc(stfwi_plan *p)
         p->ck = c.ckpt_every > 0 && c.ckpt_every < c.nt_max ? c.ckpt_every : 0;
         p->hist_nt = p->ck ? p->ck + 1 : c.nt_max;
         p->hist_cap = p->ck || c.hist_frames == 0 ? S * p->hist_nt : c.hist_frames;
-        rc |= dalloc(p, &p->hist, (size_t)p->hist_cap * 3 * p->np * es);
+        rc |= dalloc(p, &p->hist, (size_t)p->hist_cap * 3 * p->hnp * es);
         if (c.grad_params == 3) {
             p->hist2_mode = c.imaging == 2 ? 1 : 2;
             rc |= dalloc(p, &p->hist2, (size_t)p->hist_cap * 2 * p->np * es);
@@ -720,7 +729,7 @@ static int forward_t(stfwi_plan *p, void *rec, int rec_mask, void *illum, int il
     const int last0 = K ? ((p->nt_run - 1) / K) * K : 0;
     if (K && p->hist) {
         a.hist_t0 = last0 - 1;
-        fill<T>(p->hist, (i64)S * p->hist_nt * 3 * p->np, st);
+        fill<T>(p->hist, (i64)S * p->hist_nt * 3 * p->hnp, st);
     }
     hipStream_t gs[STFWI_MAX_GROUPS];
     HIP_TRY(fork_groups(p, G, st, gs));
