@@ -162,7 +162,11 @@ def test_plan_footprint_and_checkpoint_planner(lib):
     of the smallest plan (near sqrt(nt)), and refuses what cannot fit."""
     from stfwi_amd.plan import footprint, choose_ckpt
     nz, nx, pad, nt, S = 152, 500, 20, 3000, 25
-    np_ = nz * nx
+    # a history row holds the physical columns at offset pad mod 32 of a row
+    # padded to a multiple of 32 elements (a tile's columns start a line)
+    hpitch = (nx + pad % 32 + 31) // 32 * 32
+    assert hpitch == 544
+    np_ = nz * hpitch
     full = _cfg(nz_dom=nz, nx_dom=nx, pad=pad, n_shots_max=S, nt_max=nt, nrec_max=500, imaging=2,
                 store_history=1)
     b_full = footprint(full)
