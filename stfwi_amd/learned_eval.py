@@ -50,9 +50,9 @@ CUTOFFS = (3.0, 5.0, 8.0)
 FD8_S = 1225 / 1024 + 245 / 3072 + 49 / 5120 + 5 / 7168
 
 
-def stages(iters=(20, 20, 20), f_c=8.0):
+def stages(iters=(20, 20, 20), f_c=8.0, cutoffs=CUTOFFS):
     return [dict(f_c=f_c, cutoff=c, n_iters=int(n), invert="vs", n_taper_pct=0.1, precon="illum",
-                 illumination_eps=1e-4, grad_smooth_sigma=1.0) for c, n in zip(CUTOFFS, iters)]
+                 illumination_eps=1e-4, grad_smooth_sigma=1.0) for c, n in zip(cutoffs, iters)]
 
 
 def survey(seed):
@@ -146,11 +146,17 @@ class _Track:
         return False
 
 
-def evaluate_seed(seed, nets=None, plain_iters=(20, 20, 20), log=None):
+def evaluate_seed(seed, nets=None, plain_iters=(20, 20, 20), log=None, schedule=None, ablation=False):
     """Plain run, then one learned run per entry of `nets` ({name: lp}; each
     lp carries its own policy and maxls) stopped at the plain final Vs
     RMSE.  Every run keeps its RMSE trace: (stage, iteration, seconds,
-    rmse) after each accepted iteration."""
+    rmse) after each accepted iteration.
+
+    schedule = (cutoffs, iters) gives the learned runs bands of their own
+    (the plain run keeps the 3/5/8 Hz bands); ablation=True then also runs
+    the plain method on that schedule, without the network ('plain on the
+    schedule'), so that what the schedule alone gains is measured beside
+    what the network adds."""
     sv = survey(seed)
     obs = observed_data(sv)
     rm0 = vs_rmse(torch.as_tensor(sv["init"][0]), torch.as_tensor(sv["true"][0], dtype=torch.float64))
@@ -165,10 +171,14 @@ def evaluate_seed(seed, nets=None, plain_iters=(20, 20, 20), log=None):
                plain_improves=rp < rm0, plain_trace=_rounded(tr.trace), learned={})
     if log is not None:
         log(f"seed {seed}: plain RMSE {rm0:.2f} -> {rp:.2f} m/s in {t_plain:.1f} s, {out['plain_evals']} evaluations")
-    for name, lp in (nets or {}).items():
+    runs = dict(nets or {})
+    if schedule is not None and ablation:
+        runs["plain on the schedule"] = None
+    st_l = stages(schedule[1], cutoffs=schedule[0]) if schedule is not None else stages(plain_iters)
+    for name, lp in runs.items():
         tl = _Track(sv["true"][0], target=rp)
-        _, hl = invert(sv, obs, stages(plain_iters), learned=lp, monitor=tl)
-        r = dict(policy=getattr(lp, "policy", "every"), maxls=getattr(lp, "maxls", None),
+        _, hl = invert(sv, obs, st_l, learned=lp, monitor=tl)
+        r = dict(policy=getattr(lp, "policy", None), maxls=getattr(lp, "maxls", None),
                  reached=tl.hit is not None, seconds=tl.hit[2] if tl.hit is not None else None,
                  run_s=tl.trace[-1][2], rmse_final=tl.trace[-1][3], iters=sum(h["n_iters_done"] for h in hl),
                  evals=sum(h["n_evals"] for h in hl), learned_steps=sum(h["n_learned"] for h in hl),
@@ -229,6 +239,10 @@ def main(argv=None):
     ap.add_argument("--seeds", required=True)
     ap.add_argument("--modes", default="final")
     ap.add_argument("--variants", default="final:every:10")
+    ap.add_argument("--schedule", default=None,
+                    help="evaluate: bands of the learned runs, e.g. 8:40 or 5,8:10,30 (cutoffs:iterations)")
+    ap.add_argument("--ablation", action="store_true",
+                    help="evaluate: also run the plain method on --schedule, without the network")
     ap.add_argument("--gen-iters", default="2,2,6",
                     help="generate-train: plain iterations in the 3, 5 and 8 Hz bands of each training run")
     ap.add_argument("--epochs", type=int, default=20)
@@ -264,9 +278,13 @@ def main(argv=None):
             lp.save(os.path.join(args.out, f"{mode}.pt"))
             log(f"{mode}: {len(col.pairs[mode])} pairs, loss {h[0]:.4e} -> {h[-1]:.4e}")
     else:
-        nets = {v: variant(args.weights, v) for v in args.variants.split(",")}
+        nets = {v: variant(args.weights, v) for v in args.variants.split(",")} if args.variants else {}
+        sched = None
+        if args.schedule:
+            cs, its = args.schedule.split(":")
+            sched = (tuple(float(c) for c in cs.split(",")), tuple(int(n) for n in its.split(",")))
         for seed in _seeds(args.seeds):
-            r = evaluate_seed(seed, nets, log=log)
+            r = evaluate_seed(seed, nets, log=log, schedule=sched, ablation=args.ablation)
             with open(args.out, "a") as f:
                 f.write(json.dumps(r) + "\n")
 
