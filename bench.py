@@ -342,7 +342,7 @@ def main():
     ap.add_argument("--seeds", default="2000-2015", help="held-out seeds for --learned-vs-plain (a-b)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "stfwi_amd", "weights"),
                     help="directory of the trained preconditioner (<mode>.pt)")
-    ap.add_argument("--variant", default="final:until_fail:1",
+    ap.add_argument("--variant", default="final:stage:2",
                     help="network target and how the driver uses it, mode:policy:maxls (--learned-vs-plain)")
     ap.add_argument("--learned", action="store_true",
                     help="time the learned preconditioner's matrix-core convolution kernels and one "
