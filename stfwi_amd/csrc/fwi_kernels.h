@@ -441,7 +441,8 @@ __device__ __forceinline__ void logical_block(const UDiv &dgx, int &bx, int &bz)
     const int L = blockIdx.x + gx * blockIdx.z;
     const int q = n >> 3, rmd = n & 7;
     const int xcd = L & 7, w = L >> 3;
-    const int lg = xcd < rmd ? xcd * (q + 1) + w : rmd * (q + 1) + (xcd - rmd) * q + w;
+    // XCDs below rmd hold q + 1 logical blocks, the others q (no branch)
+    const int lg = xcd * q + min(xcd, rmd) + w;
     bz = (int)udiv((unsigned)lg, dgx);
     bx = lg - bz * gx;
 }
@@ -511,7 +512,6 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
         if (a.rec_mask & 8) ro[(i64)3 * a.nt_max * a.nrec_max] = st[3 * g.fstride + cell];
         return;
     }
-    if (it >= sh.nt) return;
     constexpr int RPT = TZ / BLOCK_Y;
     __shared__ T sP1[TZ + 1][TILE_X + 8], sP2[TZ + 1][TILE_X + 8], sP3x[TZ + 1][TILE_X + 8];
     __shared__ T sP3z[TZ + 8][TILE_X], sP4[TZ + 8][TILE_X], sP5[TZ + 8][TILE_X];
@@ -622,6 +622,10 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
             sP3x[zi][xi] = xok[k] ? T(2) * mu * xs[k] : T(0);
         }
     }
+    // a shot past its record returns here, after its staging loads were
+    // issued (they stay inside the shot's buffers at any step), so that
+    // the shot-table lookup overlaps them; the test is block-uniform
+    if (it >= sh.nt) return;
     __syncthreads();
     auto body = [&](auto tagx, auto tagz) {
         constexpr bool IX = decltype(tagx)::value, IZ = decltype(tagz)::value;
@@ -713,10 +717,9 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
     logical_block(a.dgx, bx, s);
     s += a.shot0;
     const ShotDev sh = a.shots[s];
-    if (it >= sh.nt) return;
     if (bx >= a.ntiles) {
         const int j = (bx - a.ntiles) * 256 + threadIdx.y * TILE_X + threadIdx.x;
-        if (j >= sh.nrec || !a.rec) return;
+        if (it >= sh.nt || j >= sh.nrec || !a.rec) return;
         const i64 cell = g.origin + a.rec_cell[sh.rec_off + j];
         const T *st = a.state + s * a.sstride;
         T *ro = a.rec + (i64)s * 4 * a.nt_max * a.nrec_max + (i64)it * a.nrec_max + j;
@@ -812,6 +815,10 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
             sV[zi][xi] = xok[k] ? xv[k] : T(0);
         }
     }
+    // a shot past its record returns here, after its staging loads were
+    // issued (they stay inside the shot's buffers at any step), so that
+    // the shot-table lookup overlaps them; the test is block-uniform
+    if (it >= sh.nt) return;
     __syncthreads();
     auto body = [&](auto tagx, auto tagz) {
         constexpr bool IX = decltype(tagx)::value, IZ = decltype(tagz)::value;
@@ -1082,7 +1089,6 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
     logical_block(a.dgx, bx, s);
     s += a.shot0;
     const ShotDev sh = a.shots[s];
-    if (it >= sh.nt) return;
     constexpr int RPT = TZ / BLOCK_Y;
     __shared__ T szU[TZ + 8][TILE_X], szV[TZ + 8][TILE_X];
     __shared__ T sxV[TZ + 1][TILE_X + 8], sxU[TZ + 1][TILE_X + 8];
@@ -1273,6 +1279,10 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
             }
         }
     }
+    // a shot past its record returns here, after its staging loads were
+    // issued (they stay inside the shot's buffers at any step), so that
+    // the shot-table lookup overlaps them; the test is block-uniform
+    if (it >= sh.nt) return;
     __syncthreads();
     auto body = [&](auto tagx, auto tagz) {
         constexpr bool IX = decltype(tagx)::value, IZ = decltype(tagz)::value;
@@ -1437,7 +1447,6 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
     logical_block(a.dgx, bx, s);
     s += a.shot0;
     const ShotDev sh = a.shots[s];
-    if (it >= sh.nt) return;
     constexpr int RPT = TZ / BLOCK_Y;
     __shared__ T sxBU[TZ + 1][TILE_X + 8], sxMes[TZ + 1][TILE_X + 8];
     __shared__ T szBV[TZ + 8][TILE_X], szMes[TZ + 8][TILE_X];
@@ -1642,9 +1651,13 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
     }
     };
     STFWI_DISPATCH(inx, inz, stage);
-    __syncthreads();
-    // history frame and accumulators, issued after the barrier: their
-    // registers are not live across the staging
+    // a shot past its record returns here, after its staging loads were
+    // issued (they stay inside the shot's buffers at any step), so that
+    // the shot-table lookup overlaps them; the test is block-uniform
+    if (it >= sh.nt) return;
+    // history frame and accumulators, issued after the staging (their
+    // registers are not live across it) and before the barrier, whose
+    // wait hides their latency
 #pragma unroll
     for (int rr = 0; rr < RPT; ++rr) {
         const int r = r0 + ty + BLOCK_Y * rr;
@@ -1667,6 +1680,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
             if (a.gparams >= 2) ac1[rr] = ldgg(accb, oa + npi);
         }
     }
+    __syncthreads();
     auto body = [&](auto tagx, auto tagz) {
         constexpr bool IX = decltype(tagx)::value, IZ = decltype(tagz)::value;
         // the column (x) term masks as per-lane weights, set once per tile
