@@ -89,7 +89,15 @@ __device__ __forceinline__ int zstrip(const Geo<T> &g, int r, int c)
 // arithmetic is emitted.  The resource's range is unbounded: every index
 // passed is >= 0 and inside the allocation.  Word 3 is the CDNA (gfx9
 // family) raw-buffer configuration.
+//
+// The wavefield-state stores (stg, stgg) carry the non-temporal hint: the
+// next kernel reads the state after the kernel boundary, whose acquire
+// invalidates L2, so a state line kept dirty in L2 only adds to the
+// write-back at the end of the kernel.  (The C-PML memories and gradient
+// accumulators keep plain stores: with the hint they were slower.)
 #define STFWI_BUF_WORD3 0x00020000
+// cache-policy operand of the buffer stores: non-temporal
+#define STFWI_BUF_NT 2
 
 template <typename T>
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const T *base)
@@ -109,13 +117,13 @@ __device__ __forceinline__ double buf_ld(__amdgpu_buffer_rsrc_t r, const double 
 
 __device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, float v, int idx)
 {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, idx * 4, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, idx * 4, 0, STFWI_BUF_NT);
 }
 
 __device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, double v, int idx)
 {
     typedef unsigned int u2 __attribute__((ext_vector_type(2)));
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, idx * 8, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, idx * 8, 0, STFWI_BUF_NT);
 }
 
 template <typename T>
@@ -155,7 +163,7 @@ template <typename T>
 __device__ __forceinline__ void stgg(T *base, int idx, T v)
 {
     const uint32_t q = opaque_off((uint32_t)idx * (uint32_t)sizeof(T));
-    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + q) = v;
+    __builtin_nontemporal_store(v, reinterpret_cast<T *>(reinterpret_cast<char *>(base) + q));
 }
 
 // stst: store of a write-once stream (the forward history, snapshots) with
