@@ -411,6 +411,7 @@ struct FwdArgs {
     int snap_mask, nsnap;
     int tiles_x, tiles_z, ntiles;
     int shot0;                         // first shot of this launch's group
+    int nt_min;                        // shortest record among the launch's shots
     UDiv dgx, dtx;                     // division by gridDim.x and by tiles_x
     int lean;                          // the LEAN instantiations may be used
 };
@@ -520,6 +521,10 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
         if (a.rec_mask & 8) ro[(i64)3 * a.nt_max * a.nrec_max] = st[3 * g.fstride + cell];
         return;
     }
+    // a shot past its record has no work at this step.  Only launches at or
+    // past the shortest record among their shots test it: the others
+    // issue their staging loads without waiting for the shot table
+    if (it >= a.nt_min && it >= sh.nt) return;
     constexpr int RPT = TZ / BLOCK_Y;
     __shared__ T sP1[TZ + 1][TILE_X + 8], sP2[TZ + 1][TILE_X + 8], sP3x[TZ + 1][TILE_X + 8];
     __shared__ T sP3z[TZ + 8][TILE_X], sP4[TZ + 8][TILE_X], sP5[TZ + 8][TILE_X];
@@ -630,10 +635,6 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
             sP3x[zi][xi] = xok[k] ? T(2) * mu * xs[k] : T(0);
         }
     }
-    // a shot past its record returns here, after its staging loads were
-    // issued (they stay inside the shot's buffers at any step), so that
-    // the shot-table lookup overlaps them; the test is block-uniform
-    if (it >= sh.nt) return;
     __syncthreads();
     auto body = [&](auto tagx, auto tagz) {
         constexpr bool IX = decltype(tagx)::value, IZ = decltype(tagz)::value;
@@ -735,6 +736,10 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
         if (a.rec_mask & 2) ro[(i64)a.nt_max * a.nrec_max] = st[g.fstride + cell];
         return;
     }
+    // a shot past its record has no work at this step.  Only launches at or
+    // past the shortest record among their shots test it: the others
+    // issue their staging loads without waiting for the shot table
+    if (it >= a.nt_min && it >= sh.nt) return;
     constexpr int RPT = TZ / BLOCK_Y;
     __shared__ T sU[TZ + 9][TILE_X + 8], sV[TZ + 9][TILE_X + 8];
     constexpr int NK = FDC<FD>::NK;
@@ -823,10 +828,6 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
             sV[zi][xi] = xok[k] ? xv[k] : T(0);
         }
     }
-    // a shot past its record returns here, after its staging loads were
-    // issued (they stay inside the shot's buffers at any step), so that
-    // the shot-table lookup overlaps them; the test is block-uniform
-    if (it >= sh.nt) return;
     __syncthreads();
     auto body = [&](auto tagx, auto tagz) {
         constexpr bool IX = decltype(tagx)::value, IZ = decltype(tagz)::value;
@@ -966,6 +967,7 @@ struct AdjArgs {
     int aen_comp;
     int tiles_x, tiles_z, ntiles;
     int shot0;                         // first shot of this launch's group
+    int nt_min;                        // shortest record among the launch's shots
     UDiv dgx, dtx;                     // division by gridDim.x and by tiles_x
     int lean;                          // the LEAN instantiations may be used
 };
@@ -1097,6 +1099,10 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
     logical_block(a.dgx, bx, s);
     s += a.shot0;
     const ShotDev sh = a.shots[s];
+    // a shot past its record has no work at this step.  Only launches at or
+    // past the shortest record among their shots test it: the others
+    // issue their staging loads without waiting for the shot table
+    if (it >= a.nt_min && it >= sh.nt) return;
     constexpr int RPT = TZ / BLOCK_Y;
     __shared__ T szU[TZ + 8][TILE_X], szV[TZ + 8][TILE_X];
     __shared__ T sxV[TZ + 1][TILE_X + 8], sxU[TZ + 1][TILE_X + 8];
@@ -1287,10 +1293,6 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
             }
         }
     }
-    // a shot past its record returns here, after its staging loads were
-    // issued (they stay inside the shot's buffers at any step), so that
-    // the shot-table lookup overlaps them; the test is block-uniform
-    if (it >= sh.nt) return;
     __syncthreads();
     auto body = [&](auto tagx, auto tagz) {
         constexpr bool IX = decltype(tagx)::value, IZ = decltype(tagz)::value;
@@ -1455,6 +1457,10 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
     logical_block(a.dgx, bx, s);
     s += a.shot0;
     const ShotDev sh = a.shots[s];
+    // a shot past its record has no work at this step.  Only launches at or
+    // past the shortest record among their shots test it: the others
+    // issue their staging loads without waiting for the shot table
+    if (it >= a.nt_min && it >= sh.nt) return;
     constexpr int RPT = TZ / BLOCK_Y;
     __shared__ T sxBU[TZ + 1][TILE_X + 8], sxMes[TZ + 1][TILE_X + 8];
     __shared__ T szBV[TZ + 8][TILE_X], szMes[TZ + 8][TILE_X];
@@ -1659,10 +1665,6 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
     }
     };
     STFWI_DISPATCH(inx, inz, stage);
-    // a shot past its record returns here, after its staging loads were
-    // issued (they stay inside the shot's buffers at any step), so that
-    // the shot-table lookup overlaps them; the test is block-uniform
-    if (it >= sh.nt) return;
     // history frame and accumulators, issued after the staging (their
     // registers are not live across it) and before the barrier, whose
     // wait hides their latency

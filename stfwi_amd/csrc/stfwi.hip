@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdarg>
@@ -587,6 +588,15 @@ static int group_bounds(const stfwi_plan *p, int *g0)
     return G;
 }
 
+// shortest record among shots [s0, s1): the time-step kernels of a launch
+// test each block's shot for a finished record only from that step on
+static int nt_min_of(const stfwi_plan *p, int s0, int s1)
+{
+    int m = INT_MAX;
+    for (int s = s0; s < s1; ++s) m = std::min(m, p->shot_nt[s]);
+    return m;
+}
+
 static hipError_t fork_groups(stfwi_plan *p, int G, hipStream_t st, hipStream_t *gs)
 {
     if (G == 1) {
@@ -736,6 +746,7 @@ static int forward_t(stfwi_plan *p, void *rec, int rec_mask, void *illum, int il
     for (int it = 0; it < p->nt_run; ++it)
         for (int k = 0; k < G; ++k) {
             a.shot0 = g0[k];
+            a.nt_min = nt_min_of(p, g0[k], g0[k + 1]);
             FwdArgs<T> b = a;
             if (K && p->hist) {
                 if (it > 0 && it % K == 0) {
@@ -756,6 +767,7 @@ static int forward_t(stfwi_plan *p, void *rec, int rec_mask, void *illum, int il
     if (rec)
         for (int k = 0; k < G; ++k) {
             a.shot0 = g0[k];
+            a.nt_min = nt_min_of(p, g0[k], g0[k + 1]);
             launch_fwd<T, FD>(0, p->tz[0], a, rblk, g0[k + 1] - g0[k], gs[k], p->nt_run);
         }
     HIP_TRY(hipGetLastError());
@@ -905,6 +917,7 @@ static int adjoint_t(stfwi_plan *p, const void *rec_adj, void *src_adj, void *sn
                 for (int it = t_lo; it < t_hi; ++it)
                     for (int k = 0; k < G; ++k) {
                         f.shot0 = g0[k];
+                        f.nt_min = nt_min_of(p, g0[k], g0[k + 1]);
                         launch_fwd<T, FD>(0, p->tz[0], f, 0, g0[k + 1] - g0[k], gs[k], it);
                         launch_fwd<T, FD>(1, p->tz[1], f, 0, g0[k + 1] - g0[k], gs[k], it);
                     }
@@ -917,6 +930,7 @@ static int adjoint_t(stfwi_plan *p, const void *rec_adj, void *src_adj, void *sn
             for (int k = 0; k < G; ++k) {
                 const int ns = g0[k + 1] - g0[k];
                 a.shot0 = g0[k];
+                a.nt_min = nt_min_of(p, g0[k], g0[k + 1]);
                 bool first = false;
                 for (int s = g0[k]; s < g0[k + 1]; ++s) first |= p->shot_nt[s] - 1 == it;
                 if (first)
@@ -1075,6 +1089,7 @@ static int time_kernel_t(stfwi_plan *p, int which, int tz, int reps, hipStream_t
         for (int r = 0; r < reps; ++r)
             for (int k = 0; k < G; ++k) {
                 a.shot0 = g0[k];
+                a.nt_min = nt_min_of(p, g0[k], g0[k + 1]);
                 launch_fwd<T, FD>(which, tz, a, 0, g0[k + 1] - g0[k], gs[k], it);
             }
         HIP_TRY(join_groups(p, G, st));
@@ -1089,6 +1104,7 @@ static int time_kernel_t(stfwi_plan *p, int which, int tz, int reps, hipStream_t
         for (int r = 0; r < reps; ++r)
             for (int k = 0; k < G; ++k) {
                 a.shot0 = g0[k];
+                a.nt_min = nt_min_of(p, g0[k], g0[k + 1]);
                 launch_adj<T, FD>(which, tz, a, g0[k + 1] - g0[k], gs[k], it);
             }
         HIP_TRY(join_groups(p, G, st));
