@@ -509,8 +509,8 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
     int bx, s;
     logical_block(a.dgx, bx, s);
     s += a.shot0;
+    const ShotDev sh = a.shots[s];
     if (bx >= a.ntiles) {
-        const ShotDev sh = a.shots[s];
         const int j = (bx - a.ntiles) * 256 + threadIdx.y * TILE_X + threadIdx.x;
         const int itp = it - 1;
         if (itp < 0 || itp >= sh.nt || j >= sh.nrec || !a.rec) return;
@@ -524,7 +524,7 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
     // a shot past its record has no work at this step.  Only launches at or
     // past the shortest record among their shots test it: the others
     // issue their staging loads without waiting for the shot table
-    if (it >= a.nt_min && it >= a.shots[s].nt) return;
+    if (it >= a.nt_min && it >= sh.nt) return;
     constexpr int RPT = TZ / BLOCK_Y;
     __shared__ T sP1[TZ + 1][TILE_X + 8], sP2[TZ + 1][TILE_X + 8], sP3x[TZ + 1][TILE_X + 8];
     __shared__ T sP3z[TZ + 8][TILE_X], sP4[TZ + 8][TILE_X], sP5[TZ + 8][TILE_X];
@@ -635,11 +635,6 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
             sP3x[zi][xi] = xok[k] ? T(2) * mu * xs[k] : T(0);
         }
     }
-    // the shot's fields used from here on, loaded after the staging loads
-    // are issued: the empty asm (no instruction) keeps the scalar load
-    // below them, so no scalar wait before the staging covers it
-    asm volatile("" ::: "memory");
-    const ShotDev sh = a.shots[s];
     __syncthreads();
     auto body = [&](auto tagx, auto tagz) {
         constexpr bool IX = decltype(tagx)::value, IZ = decltype(tagz)::value;
@@ -730,8 +725,8 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
     int bx, s;
     logical_block(a.dgx, bx, s);
     s += a.shot0;
+    const ShotDev sh = a.shots[s];
     if (bx >= a.ntiles) {
-        const ShotDev sh = a.shots[s];
         const int j = (bx - a.ntiles) * 256 + threadIdx.y * TILE_X + threadIdx.x;
         if (it >= sh.nt || j >= sh.nrec || !a.rec) return;
         const i64 cell = g.origin + a.rec_cell[sh.rec_off + j];
@@ -744,7 +739,7 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
     // a shot past its record has no work at this step.  Only launches at or
     // past the shortest record among their shots test it: the others
     // issue their staging loads without waiting for the shot table
-    if (it >= a.nt_min && it >= a.shots[s].nt) return;
+    if (it >= a.nt_min && it >= sh.nt) return;
     constexpr int RPT = TZ / BLOCK_Y;
     __shared__ T sU[TZ + 9][TILE_X + 8], sV[TZ + 9][TILE_X + 8];
     constexpr int NK = FDC<FD>::NK;
@@ -833,11 +828,6 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
             sV[zi][xi] = xok[k] ? xv[k] : T(0);
         }
     }
-    // the shot's fields used from here on, loaded after the staging loads
-    // are issued: the empty asm (no instruction) keeps the scalar load
-    // below them, so no scalar wait before the staging covers it
-    asm volatile("" ::: "memory");
-    const ShotDev sh = a.shots[s];
     __syncthreads();
     auto body = [&](auto tagx, auto tagz) {
         constexpr bool IX = decltype(tagx)::value, IZ = decltype(tagz)::value;
@@ -1108,10 +1098,11 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
     int bx, s;
     logical_block(a.dgx, bx, s);
     s += a.shot0;
+    const ShotDev sh = a.shots[s];
     // a shot past its record has no work at this step.  Only launches at or
     // past the shortest record among their shots test it: the others
     // issue their staging loads without waiting for the shot table
-    if (it >= a.nt_min && it >= a.shots[s].nt) return;
+    if (it >= a.nt_min && it >= sh.nt) return;
     constexpr int RPT = TZ / BLOCK_Y;
     __shared__ T szU[TZ + 8][TILE_X], szV[TZ + 8][TILE_X];
     __shared__ T sxV[TZ + 1][TILE_X + 8], sxU[TZ + 1][TILE_X + 8];
@@ -1302,11 +1293,6 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
             }
         }
     }
-    // the shot's fields used from here on, loaded after the staging loads
-    // are issued: the empty asm (no instruction) keeps the scalar load
-    // below them, so no scalar wait before the staging covers it
-    asm volatile("" ::: "memory");
-    const ShotDev sh = a.shots[s];
     __syncthreads();
     auto body = [&](auto tagx, auto tagz) {
         constexpr bool IX = decltype(tagx)::value, IZ = decltype(tagz)::value;
@@ -1470,10 +1456,11 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
     int bx, s;
     logical_block(a.dgx, bx, s);
     s += a.shot0;
+    const ShotDev sh = a.shots[s];
     // a shot past its record has no work at this step.  Only launches at or
     // past the shortest record among their shots test it: the others
     // issue their staging loads without waiting for the shot table
-    if (it >= a.nt_min && it >= a.shots[s].nt) return;
+    if (it >= a.nt_min && it >= sh.nt) return;
     constexpr int RPT = TZ / BLOCK_Y;
     __shared__ T sxBU[TZ + 1][TILE_X + 8], sxMes[TZ + 1][TILE_X + 8];
     __shared__ T szBV[TZ + 8][TILE_X], szMes[TZ + 8][TILE_X];
@@ -1525,6 +1512,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
     // owned values, loaded ahead of the barrier
     T l2m0[RPT], mu0[RPT], exa0v[RPT], eza0v[RPT], esa0v[RPT], h0[RPT], h1[RPT], h2v[RPT], ac0[RPT], ac1[RPT];
     const i64 hs = (i64)s * a.nt_max;            // wavelet / dxu0 rows
+    const i64 hh = (i64)sh.hoff - a.hist_t0;      // history frame base
     const int hframe = a.imaging == 2 ? it - 1 : it;
 #pragma unroll
     for (int rr = 0; rr < RPT; ++rr) {
@@ -1677,12 +1665,6 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
     }
     };
     STFWI_DISPATCH(inx, inz, stage);
-    // the shot's fields used from here on, loaded after the staging loads
-    // are issued: the empty asm (no instruction) keeps the scalar load
-    // below them, so no scalar wait before the staging covers it
-    asm volatile("" ::: "memory");
-    const ShotDev sh = a.shots[s];
-    const i64 hh = (i64)sh.hoff - a.hist_t0;      // history frame base
     // history frame and accumulators, issued after the staging (their
     // registers are not live across it) and before the barrier, whose
     // wait hides their latency
