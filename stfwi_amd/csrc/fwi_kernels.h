@@ -93,8 +93,10 @@ __device__ __forceinline__ int zstrip(const Geo<T> &g, int r, int c)
 // The wavefield-state stores (stg, stgg) carry the non-temporal hint: the
 // next kernel reads the state after the kernel boundary, whose acquire
 // invalidates L2, so a state line kept dirty in L2 only adds to the
-// write-back at the end of the kernel.  (The C-PML memories and gradient
-// accumulators keep plain stores: with the hint they were slower.)
+// write-back at the end of the kernel.  The exact-imaging gradient
+// accumulators, whose rows share the history's line-aligned layout, are
+// stored the same way; the C-PML memories keep plain stores (their strips
+// start mid-line, and with the hint they were slower).
 #define STFWI_BUF_WORD3 0x00020000
 // cache-policy operand of the buffer stores: non-temporal
 #define STFWI_BUF_NT 2
@@ -960,7 +962,7 @@ struct AdjArgs {
     const T *hist, *hist2;             // [frame][3][hnp] / [frame][2][np], shot s from frame hoff
     int hist_nt, hist_t0;              // frames per shot with checkpoints; step of frame 0
     const T *dxu0;
-    T *gacc;                           // [shot][nacc][np]
+    T *gacc;                           // [shot][nacc][hnp]: rows of hpitch, as the history
     T *snap;
     int snap_mask, nsnap;
     T *aen;                            // [shot][nz*nx]: sum over steps of the squared adjoint field aen_comp
@@ -1679,15 +1681,15 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
             const bool pok = rp >= 0 && rp < g.nzd && cp >= 0 && cp < g.nxd;
             const bool hok = hframe >= 0 && pok;
             const T *hfr = a.hist + (hh + (hframe >= a.hist_t0 ? hframe : a.hist_t0)) * 3 * g.hnp;
-            const int o = hok ? rp * g.hpitch + cp + g.hcoff : 0, nph = (int)g.hnp, npi = (int)g.np;
+            const int o = hok ? rp * g.hpitch + cp + g.hcoff : 0, nph = (int)g.hnp;
             h0[rr] = ldgg(hfr, o);
             h1[rr] = ldgg(hfr, o + nph);
             h2v[rr] = ldgg(hfr, o + 2 * nph);
-            // first two gradient accumulators of the cell
-            const T *accb = a.gacc + (i64)s * a.nacc * g.np;
-            const int oa = pok ? rp * g.nxd + cp : 0;
+            // first two gradient accumulators of the cell (the history's layout)
+            const T *accb = a.gacc + (i64)s * a.nacc * g.hnp;
+            const int oa = pok ? rp * g.hpitch + cp + g.hcoff : 0;
             ac0[rr] = ldgg(accb, oa);
-            if (a.gparams >= 2) ac1[rr] = ldgg(accb, oa + npi);
+            if (a.gparams >= 2) ac1[rr] = ldgg(accb, oa + nph);
         }
     }
     __syncthreads();
@@ -1796,7 +1798,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
             const int rp = r - g.z0, cp = c - g.pad;
             if (a.imaging == 0 || !(CZ(rp >= 0 && rp < g.nzd) && CX(cp >= 0 && cp < g.nxd))) continue;
             const int ip = rp * g.nxd + cp;
-            T *acc = a.gacc + (i64)s * a.nacc * g.np + ip;
+            T *acc = a.gacc + (i64)s * a.nacc * g.hnp + (rp * g.hpitch + cp + g.hcoff);
             if (a.imaging == 2) {
                 // exact discrete gradient: model dependence of step it, paired with
                 // its carry-in strains (history frame it-1)
@@ -1824,12 +1826,14 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
                     const T *h2 = a.hist2 + (hh + it) * 2 * g.np + ip;
                     gb = fma(Ua[q], h2[0], Va[q] * h2[g.np]);
                     acc[0] = ac0[rr] + gl2m;
-                    acc[g.np] = ac1[rr] + gl;
-                    acc[2 * g.np] += gm;
-                    acc[3 * g.np] += gb;
+                    acc[g.hnp] = ac1[rr] + gl;
+                    acc[2 * g.hnp] += gm;
+                    acc[3 * g.hnp] += gb;
                 } else {
-                    acc[0] = ac0[rr] + (gm - T(2) * gl);
-                    if (a.gparams == 2) acc[g.np] = ac1[rr] + (gl2m + gl);
+                    // a wavefront's 64 cells fill two whole 128-byte lines of the
+                    // accumulator rows: stored without keeping them in L2
+                    __builtin_nontemporal_store(ac0[rr] + (gm - T(2) * gl), acc);
+                    if (a.gparams == 2) __builtin_nontemporal_store(ac1[rr] + (gl2m + gl), acc + g.hnp);
                 }
             } else {
                 // Zhou imaging (fwi/imaging.py:125-152): forward frame it with the
@@ -1861,11 +1865,11 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
                         grl -= Ua[q] * h2[0] / i2dt + Va[q] * h2[g.np] / i2dt;
                     }
                     acc[0] = ac0[rr] + glam;
-                    acc[g.np] = ac1[rr] + gmu;
-                    acc[2 * g.np] += grl;
+                    acc[g.hnp] = ac1[rr] + gmu;
+                    acc[2 * g.hnp] += grl;
                 } else {
-                    acc[0] = ac0[rr] + (gmu - T(2) * glam);
-                    if (a.gparams == 2) acc[g.np] = ac1[rr] + glam;
+                    __builtin_nontemporal_store(ac0[rr] + (gmu - T(2) * glam), acc);
+                    if (a.gparams == 2) __builtin_nontemporal_store(ac1[rr] + glam, acc + g.hnp);
                 }
             }
         }
@@ -1895,17 +1899,20 @@ __global__ void k_hist_from_state(Geo<T> g, const T *state, i64 sstride, T *hist
     }
 }
 
-// sum of per-shot accumulators and chain rule to (Vs, Vp, rho)
+// sum of per-shot accumulators and chain rule to (Vs, Vp, rho); physical
+// cell i sits at (i / nxd) apitch + i % nxd + acoff of each accumulator
+// plane of anp elements
 template <typename T, typename C>
 __global__ void k_finalize(int n_shots, i64 np, int nacc, int imaging, int gparams, T dt,
                            const T *__restrict__ gacc, const C *__restrict__ vs, const C *__restrict__ vp,
-                           const C *__restrict__ rho, T *out)
+                           const C *__restrict__ rho, T *out, int nxd, int apitch, int acoff, i64 anp)
 {
     const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
+    const i64 ia = (i / nxd) * apitch + (i % nxd) + acoff;
     T s[4] = {0, 0, 0, 0};
     for (int sh = 0; sh < n_shots; ++sh)
-        for (int k = 0; k < nacc; ++k) s[k] += gacc[((i64)sh * nacc + k) * np + i];
+        for (int k = 0; k < nacc; ++k) s[k] += gacc[((i64)sh * nacc + k) * anp + ia];
     const C v_s = vs[i], v_p = vp[i], d = rho[i];
     T gvs = 0, gvp = 0, grho = 0;
     if (imaging == 1) {

@@ -270,7 +270,7 @@ static int plan_alloc(stfwi_plan *p)
             rc |= dalloc(p, &p->state_a, S * 5 * p->fstride * es);
         }
         if (c.imaging == 2 && c.free_surface) rc |= dalloc(p, &p->dxu0, (size_t)S * c.nt_max * p->nx * es);
-        rc |= dalloc(p, &p->gacc, (size_t)S * p->nacc * p->np * es);
+        rc |= dalloc(p, &p->gacc, (size_t)S * p->nacc * p->hnp * es);
     }
     rc |= dalloc(p, (void **)&p->shots_d, S * sizeof(ShotDev));
     rc |= dalloc(p, &p->rec_int, (size_t)S * 4 * c.nt_max * std::max(c.nrec_max, 1) * es);
@@ -880,7 +880,7 @@ static int adjoint_t(stfwi_plan *p, const void *rec_adj, void *src_adj, void *sn
         fill<T>(p->psix_a[k], (i64)S * 4 * p->psix_len, st);
         fill<T>(p->psiz_a[k], (i64)S * 4 * p->psiz_len, st);
     }
-    if (a.imaging) fill<T>(p->gacc, (i64)S * p->nacc * p->np, st);
+    if (a.imaging) fill<T>(p->gacc, (i64)S * p->nacc * p->hnp, st);
     // merged receiver adjoints of every step, read by the time-step kernels
     hipLaunchKernelGGL(k_merge_inj<T>, dim3((p->max_nuc + 127) / 128, std::min(p->nt_run, 65535), S), dim3(128), 0,
                        st, a, p->nt_run);
@@ -996,7 +996,7 @@ static int finalize_t(stfwi_plan *p, const void *vs, const void *vp, const void 
     const int blocks = (int)((p->np + 255) / 256);
     hipLaunchKernelGGL((k_finalize<T, C>), dim3(blocks), dim3(256), 0, st, p->n_shots, p->np, p->nacc,
                        p->cfg.imaging, p->cfg.grad_params, (T)p->cfg.dt, (const T *)p->gacc, (const C *)vs,
-                       (const C *)vp, (const C *)rho, (T *)out);
+                       (const C *)vp, (const C *)rho, (T *)out, p->cfg.nx_dom, p->hpitch, p->hcoff, p->hnp);
     HIP_TRY(hipGetLastError());
     return 0;
 }
