@@ -30,7 +30,7 @@ HBM_PEAK = 8.0e12
 # by tile height, from rocprofv3 FETCH_SIZE and WRITE_SIZE passes over
 # `bench.py --kernels-only --tiles 8,8,16,8` and `--tiles 8,4,8,4`, scaled by
 # the counters' readings for a 256 MiB device copy (reads x 2.00, writes x 1.00).
-ADJ_VEL_TRAFFIC = {4: (51545.2 * 2.0 + 38292.7) * 1024, 8: (51759.9 * 2.0 + 38354.5) * 1024}
+ADJ_VEL_TRAFFIC = {4: (51545.2 * 2.0 + 38292.7) * 1024, 8: (51842.6 * 2.0 + 38380.1) * 1024}
 
 
 def algorithmic_bytes_per_shot_gradient(s=4, k=1, nt=NT):
