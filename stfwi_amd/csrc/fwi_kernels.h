@@ -6,9 +6,10 @@
 //     written, so stencil loads at +-4 cells stay inside the allocation and
 //     read zeros there.  `origin` = 4*ld is the offset of cell (0,0).
 //   * C-PML memories exist only in the PML strips: x-strips are nz x 2*pad
-//     (left pad columns, then right pad columns), z-strips are zrows x nx
-//     (bottom pad rows first, then the top pad rows when there is no free
-//     surface).  Outside the strips the reference's recursion is an exact
+//     (left pad columns, then right pad columns), z-strips are zrows rows of
+//     ld elements (the field pitch, so that a tile's 64 columns are two whole
+//     128-byte lines; bottom pad rows first, then the top pad rows when
+//     there is no free surface).  Outside the strips the reference's recursion is an exact
 //     no-op (b = 1, a = 0, kappa = 1, psi stays 0).
 //   * histories keep the physical cells only: the strain history as
 //     [frame][field][nz_dom][hpitch] with physical column cp at cp + hcoff
@@ -78,8 +79,8 @@ __device__ __forceinline__ int xstrip(const Geo<T> &g, int r, int c)
 template <typename T>
 __device__ __forceinline__ int zstrip(const Geo<T> &g, int r, int c)
 {
-    if (r >= g.nz - g.pad) return (r - (g.nz - g.pad)) * g.nx + c;
-    if (!g.fs && r < g.pad) return (g.pad + r) * g.nx + c;
+    if (r >= g.nz - g.pad) return (r - (g.nz - g.pad)) * g.ld + c;
+    if (!g.fs && r < g.pad) return (g.pad + r) * g.ld + c;
     return -1;
 }
 
@@ -94,9 +95,10 @@ __device__ __forceinline__ int zstrip(const Geo<T> &g, int r, int c)
 // next kernel reads the state after the kernel boundary, whose acquire
 // invalidates L2, so a state line kept dirty in L2 only adds to the
 // write-back at the end of the kernel.  The exact-imaging gradient
-// accumulators, whose rows share the history's line-aligned layout, are
-// stored the same way; the C-PML memories keep plain stores (their strips
-// start mid-line, and with the hint they were slower).
+// accumulators (rows in the history's line-aligned layout) and the z-strip
+// C-PML memories (rows of the field pitch) are stored the same way; the
+// x-strip memories keep plain stores (a tile row holds 20 of their cells,
+// less than a line, and with the hint they were slower).
 #define STFWI_BUF_WORD3 0x00020000
 // cache-policy operand of the buffer stores: non-temporal
 #define STFWI_BUF_NT 2
@@ -266,6 +268,15 @@ __device__ __forceinline__ T pml_fwd_apply(T D, T psi_old, T *psi, const PmlFwd<
 {
     const T pn = fma(k.b, psi_old, k.a * D);
     *psi = pn;
+    return D * k.r + pn;
+}
+
+// the same for a z-strip memory, stored with the non-temporal hint
+template <typename T>
+__device__ __forceinline__ T pml_fwd_apply_nt(T D, T psi_old, T *psi, const PmlFwd<T> &k)
+{
+    const T pn = fma(k.b, psi_old, k.a * D);
+    __builtin_nontemporal_store(pn, psi);
     return D * k.r + pn;
 }
 
@@ -683,9 +694,9 @@ __global__ __launch_bounds__(256) void k_fwd_vel(FwdArgs<T> a, int it)
                 if (iz >= 0) {
                     T *pz = a.psiz + s * a.psiz_stride + iz;
                     const PmlFwd<T> kz = pml_fwd_coef(a.profz, r, nz);
-                    D3 = pml_fwd_apply(D3, mz[rr][0], pz + 0 * a.psiz_n, kz);
-                    D5 = pml_fwd_apply(D5, mz[rr][1], pz + 1 * a.psiz_n, kz);
-                    D6 = pml_fwd_apply(D6, mz[rr][2], pz + 2 * a.psiz_n, kz);
+                    D3 = pml_fwd_apply_nt(D3, mz[rr][0], pz + 0 * a.psiz_n, kz);
+                    D5 = pml_fwd_apply_nt(D5, mz[rr][1], pz + 1 * a.psiz_n, kz);
+                    D6 = pml_fwd_apply_nt(D6, mz[rr][2], pz + 2 * a.psiz_n, kz);
                 }
             }
             const T b = b0[rr];
@@ -870,7 +881,7 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
                                (sV[zc + k - 1][xc] - sV[zc - k][xc]);
                 }
             }
-            if (!IZ && iz >= 0) dzv = pml_fwd_apply(dzv, mz3[rr], pz + 3 * a.psiz_n, kz);
+            if (!IZ && iz >= 0) dzv = pml_fwd_apply_nt(dzv, mz3[rr], pz + 3 * a.psiz_n, kz);
             T ezn = ez0[rr] + g.dtdz * dzv;
             if (sh.type == 1 && r == sh.sz && c == sh.sx) {
                 const T lm = (a.L2M[g.origin + q] - T(2) * a.M[g.origin + q]) + a.M[g.origin + q];
@@ -892,7 +903,7 @@ __global__ __launch_bounds__(256) void k_fwd_strain(FwdArgs<T> a, int it)
                            (sU[zc + k][xc] - sU[zc - k + 1][xc]);
                     dxv += (CX(c >= k - 1 && c <= nx - 1 - k) ? ck : T(0)) * (sV[zc][xc + k] - sV[zc][xc - k + 1]);
                 }
-                if (!IZ && iz >= 0) dzu = pml_fwd_apply(dzu, mz4[rr], pz + 4 * a.psiz_n, kz);
+                if (!IZ && iz >= 0) dzu = pml_fwd_apply_nt(dzu, mz4[rr], pz + 4 * a.psiz_n, kz);
                 if (!IX && ix >= 0) dxv = pml_fwd_apply(dxv, mx4[rr], px + 4 * a.psix_n, kx);
                 esn = esn + T(0.5) * fma(g.dtdz, dzu, g.dtdx * dxv);
                 stgg(es, q, esn);
@@ -1358,8 +1369,8 @@ __global__ __launch_bounds__(256) void k_adj_strain(AdjArgs<T> a, int it)
                 if (iz >= 0) {
                     T *pzn = a.psiz_n + s * a.psiz_stride + iz;
                     const PmlIn<T> &m = pz[rr + 1];
-                    pzn[0] = m.b * (gzu_o[rr] + m.m0);
-                    pzn[a.psiz_len] = m.b * (gzv_o[rr] + m.m1);
+                    __builtin_nontemporal_store(m.b * (gzu_o[rr] + m.m0), pzn + 0);
+                    __builtin_nontemporal_store(m.b * (gzv_o[rr] + m.m1), pzn + a.psiz_len);
                 }
             }
             if (!IX) {
@@ -1771,8 +1782,8 @@ __global__ __launch_bounds__(256) void k_adj_vel(AdjArgs<T> a, int it)
                 if (iz >= 0) {
                     T *pzn = a.psiz_n + s * a.psiz_stride + iz;
                     const PmlIn<T> &m = pz[rr + 1];
-                    pzn[2 * a.psiz_len] = m.b * (gzbv_o[rr] + m.m0);
-                    pzn[3 * a.psiz_len] = m.b * (gzmes_o[rr] + m.m1);
+                    __builtin_nontemporal_store(m.b * (gzbv_o[rr] + m.m0), pzn + 2 * a.psiz_len);
+                    __builtin_nontemporal_store(m.b * (gzmes_o[rr] + m.m1), pzn + 3 * a.psiz_len);
                 }
             }
 

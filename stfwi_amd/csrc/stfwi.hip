@@ -236,7 +236,7 @@ static int plan_alloc(stfwi_plan *p)
     p->hpitch = (c.nx_dom + p->hcoff + 31) / 32 * 32;
     p->hnp = (i64)c.nz_dom * p->hpitch;
     p->psix_len = (i64)p->nz * 2 * c.pad;
-    p->psiz_len = (i64)p->zrows * p->nx;
+    p->psiz_len = (i64)p->zrows * p->ld;
     p->nacc = c.grad_params == 3 ? (c.imaging == 1 ? 3 : 4) : c.grad_params;
     const size_t es = p->esz;
     const i64 S = c.n_shots_max;
