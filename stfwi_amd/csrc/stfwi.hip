@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -11,6 +12,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -83,6 +85,8 @@ struct stfwi_plan {
     int groups = 1;
     // the time-step kernels' LEAN instantiations may be used (on by default)
     int lean = 1;
+    // the shot groups' time loops are enqueued by one host thread each
+    int mt_launch = 1;
     hipStream_t gstream[STFWI_MAX_GROUPS] = {};
     hipEvent_t ev_fork = nullptr, ev_join[STFWI_MAX_GROUPS] = {};
     // group k + 1 starts its time loop after group k's first kernel, so
@@ -638,6 +642,82 @@ static hipError_t join_groups(stfwi_plan *p, int G, hipStream_t st)
     return e;
 }
 
+// The time loops of the shot groups are enqueued by one host thread per
+// group (group 0 on the calling thread), so that the host's launch rate
+// does not bound the loops: every stream gets the same launches in the same
+// order as from one thread.  body(k) enqueues group k's work; the calling
+// thread's device is made current in each worker; the first failing
+// group's status and message are returned.  Threads are used only when the
+// plan allows it and the caller's stream is not being captured.
+struct GroupHandoff {
+    // group k has enqueued its first kernel (and the event that group k + 1
+    // waits on), or has stopped
+    std::atomic<int> ready[STFWI_MAX_GROUPS];
+    GroupHandoff()
+    {
+        for (auto &r : ready) r.store(0, std::memory_order_relaxed);
+    }
+    void wait_prev(int k)
+    {
+        if (k > 0)
+            while (!ready[k - 1].load(std::memory_order_acquire)) std::this_thread::yield();
+    }
+    void mark(int k) { ready[k].store(1, std::memory_order_release); }
+};
+
+template <typename F>
+static int for_groups(stfwi_plan *p, int G, hipStream_t st, F body)
+{
+    GroupHandoff h;
+    auto run = [&](int k) {
+        const int rc = body(k, h);
+        h.mark(k);          // also when body stopped early: no group waits forever
+        return rc;
+    };
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    bool capturing = false;
+    if (G > 1) {
+        if (hipStreamIsCapturing(st, &cs) == hipSuccess) capturing = cs != hipStreamCaptureStatusNone;
+        else (void)hipGetLastError();   // not a capture: leave no error behind
+    }
+    const bool threads = G > 1 && p->mt_launch && !capturing;
+    if (!threads) {
+        for (int k = 0; k < G; ++k) {
+            const int rc = run(k);
+            if (rc) return rc;
+        }
+        return 0;
+    }
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    int rcs[STFWI_MAX_GROUPS] = {};
+    std::string msg[STFWI_MAX_GROUPS];
+    std::thread th[STFWI_MAX_GROUPS];
+    int spawned = 1;
+    for (int k = 1; k < G; ++k) {
+        try {
+            th[k] = std::thread([&, k] {
+                const hipError_t e = hipSetDevice(dev);
+                rcs[k] = e == hipSuccess ? run(k) : fail(STFWI_ERR_HIP, "hipSetDevice failed: %s", hipGetErrorString(e));
+                if (e != hipSuccess) h.mark(k);
+                if (rcs[k]) msg[k] = g_err;
+            });
+        } catch (...) {
+            break;          // no thread: the remaining groups run here below
+        }
+        spawned = k + 1;
+    }
+    rcs[0] = run(0);
+    for (int k = spawned; k < G; ++k) rcs[k] = run(k);
+    for (int k = 1; k < spawned; ++k) th[k].join();
+    for (int k = 0; k < G; ++k)
+        if (rcs[k]) {
+            if (k > 0 && k < spawned) g_err = msg[k];
+            return rcs[k];
+        }
+    return 0;
+}
+
 template <typename T>
 static FwdArgs<T> fwd_args(stfwi_plan *p, void *rec, int rec_mask, void *illum, int illum_comp, void *snap,
                            int snap_mask)
@@ -743,34 +823,40 @@ static int forward_t(stfwi_plan *p, void *rec, int rec_mask, void *illum, int il
     }
     hipStream_t gs[STFWI_MAX_GROUPS];
     HIP_TRY(fork_groups(p, G, st, gs));
-    for (int it = 0; it < p->nt_run; ++it)
-        for (int k = 0; k < G; ++k) {
-            a.shot0 = g0[k];
-            a.nt_min = nt_min_of(p, g0[k], g0[k + 1]);
-            FwdArgs<T> b = a;
+    const int rc = for_groups(p, G, st, [&](int k, GroupHandoff &h) -> int {
+        FwdArgs<T> ak = a;
+        ak.shot0 = g0[k];
+        ak.nt_min = nt_min_of(p, g0[k], g0[k + 1]);
+        const int ns = g0[k + 1] - g0[k];
+        for (int it = 0; it < p->nt_run; ++it) {
+            FwdArgs<T> b = ak;
             if (K && p->hist) {
                 if (it > 0 && it % K == 0) {
-                    const int rc = ckpt_copy<T>(p, it / K, g0[k], g0[k + 1], false, gs[k]);
-                    if (rc) return rc;
+                    const int rc2 = ckpt_copy<T>(p, it / K, g0[k], g0[k + 1], false, gs[k]);
+                    if (rc2) return rc2;
                 }
                 if (it < a.hist_t0) {   // history of earlier segments is recomputed later
                     b.hist = nullptr;
                     b.hist2_mode = 0;
                 }
             }
-            if (it == 0) HIP_TRY(stagger_in(p, k, gs));
-            launch_fwd<T, FD>(0, p->tz[0], b, rblk, g0[k + 1] - g0[k], gs[k], it);
-            if (it == 0) HIP_TRY(stagger_out(p, k, G, gs));
-            launch_fwd<T, FD>(1, p->tz[1], b, rblk, g0[k + 1] - g0[k], gs[k], it);
+            if (it == 0) {
+                h.wait_prev(k);
+                HIP_TRY(stagger_in(p, k, gs));
+            }
+            launch_fwd<T, FD>(0, p->tz[0], b, rblk, ns, gs[k], it);
+            if (it == 0) {
+                HIP_TRY(stagger_out(p, k, G, gs));
+                h.mark(k);
+            }
+            launch_fwd<T, FD>(1, p->tz[1], b, rblk, ns, gs[k], it);
         }
-    // exx/ezz receivers of the last step
-    if (rec)
-        for (int k = 0; k < G; ++k) {
-            a.shot0 = g0[k];
-            a.nt_min = nt_min_of(p, g0[k], g0[k + 1]);
-            launch_fwd<T, FD>(0, p->tz[0], a, rblk, g0[k + 1] - g0[k], gs[k], p->nt_run);
-        }
-    HIP_TRY(hipGetLastError());
+        // exx/ezz receivers of the last step
+        if (rec) launch_fwd<T, FD>(0, p->tz[0], ak, rblk, ns, gs[k], p->nt_run);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    });
+    if (rc) return rc;
     HIP_TRY(join_groups(p, G, st));
     return 0;
 }
@@ -894,57 +980,61 @@ static int adjoint_t(stfwi_plan *p, const void *rec_adj, void *src_adj, void *sn
     const int nseg = (p->nt_run + K - 1) / K;
     FwdArgs<T> f = fwd_args<T>(p, nullptr, 0, nullptr, 0, nullptr, 0);
     f.dxu0 = nullptr;                 // kept from the first forward
-    for (int seg = nseg - 1; seg >= 0; --seg) {
-        const int t_lo = seg * K, t_hi = std::min(t_lo + K, p->nt_run);
-        if (p->ck && a.imaging) {
-            a.hist_t0 = f.hist_t0 = t_lo - 1;
-            if (seg < nseg - 1) {
-                for (int k = 0; k < G; ++k) {
-                    const int s0 = g0[k], ns = g0[k + 1] - g0[k];
+    const int rc = for_groups(p, G, st, [&](int k, GroupHandoff &h) -> int {
+        AdjArgs<T> ak = a;
+        FwdArgs<T> fk = f;
+        const int s0 = g0[k], ns = g0[k + 1] - g0[k];
+        ak.shot0 = fk.shot0 = s0;
+        ak.nt_min = fk.nt_min = nt_min_of(p, s0, g0[k + 1]);
+        for (int seg = nseg - 1; seg >= 0; --seg) {
+            const int t_lo = seg * K, t_hi = std::min(t_lo + K, p->nt_run);
+            if (p->ck && a.imaging) {
+                ak.hist_t0 = fk.hist_t0 = t_lo - 1;
+                if (seg < nseg - 1) {
                     if (seg == 0) {
                         fill<T>((T *)p->state + (i64)s0 * 5 * p->fstride, (i64)ns * 5 * p->fstride, gs[k]);
                         fill<T>((T *)p->psix_f + (i64)s0 * 5 * p->psix_len, (i64)ns * 5 * p->psix_len, gs[k]);
                         fill<T>((T *)p->psiz_f + (i64)s0 * 5 * p->psiz_len, (i64)ns * 5 * p->psiz_len, gs[k]);
                     } else {
-                        const int rc = ckpt_copy<T>(p, seg, s0, g0[k + 1], true, gs[k]);
-                        if (rc) return rc;
+                        const int rc2 = ckpt_copy<T>(p, seg, s0, g0[k + 1], true, gs[k]);
+                        if (rc2) return rc2;
                     }
                     const int nb = (int)std::min<i64>((p->np + 255) / 256, 1024);
-                    hipLaunchKernelGGL(k_hist_from_state<T>, dim3(nb, ns), dim3(256), 0, gs[k], f.g,
+                    hipLaunchKernelGGL(k_hist_from_state<T>, dim3(nb, ns), dim3(256), 0, gs[k], fk.g,
                                        (const T *)p->state, (i64)5 * p->fstride, (T *)p->hist,
                                        (const ShotDev *)p->shots_d, 0, s0);
-                }
-                for (int it = t_lo; it < t_hi; ++it)
-                    for (int k = 0; k < G; ++k) {
-                        f.shot0 = g0[k];
-                        f.nt_min = nt_min_of(p, g0[k], g0[k + 1]);
-                        launch_fwd<T, FD>(0, p->tz[0], f, 0, g0[k + 1] - g0[k], gs[k], it);
-                        launch_fwd<T, FD>(1, p->tz[1], f, 0, g0[k + 1] - g0[k], gs[k], it);
+                    for (int it = t_lo; it < t_hi; ++it) {
+                        launch_fwd<T, FD>(0, p->tz[0], fk, 0, ns, gs[k], it);
+                        launch_fwd<T, FD>(1, p->tz[1], fk, 0, ns, gs[k], it);
                     }
+                }
             }
-        }
-        for (int it = t_hi - 1; it >= t_lo; --it) {
-            const int cur = it & 1, prev = cur ^ 1;
-            a.psix_o = (const T *)p->psix_a[prev]; a.psiz_o = (const T *)p->psiz_a[prev];
-            a.psix_n = (T *)p->psix_a[cur]; a.psiz_n = (T *)p->psiz_a[cur];
-            for (int k = 0; k < G; ++k) {
-                const int ns = g0[k + 1] - g0[k];
-                a.shot0 = g0[k];
-                a.nt_min = nt_min_of(p, g0[k], g0[k + 1]);
+            for (int it = t_hi - 1; it >= t_lo; --it) {
+                const int cur = it & 1, prev = cur ^ 1;
+                ak.psix_o = (const T *)p->psix_a[prev]; ak.psiz_o = (const T *)p->psiz_a[prev];
+                ak.psix_n = (T *)p->psix_a[cur]; ak.psiz_n = (T *)p->psiz_a[cur];
                 bool first = false;
-                for (int s = g0[k]; s < g0[k + 1]; ++s) first |= p->shot_nt[s] - 1 == it;
+                for (int s = s0; s < g0[k + 1]; ++s) first |= p->shot_nt[s] - 1 == it;
                 if (first)
-                    hipLaunchKernelGGL(k_adj_inject<T>, dim3((p->max_nuc + 63) / 64, 1, ns), dim3(64), 0, gs[k], a,
+                    hipLaunchKernelGGL(k_adj_inject<T>, dim3((p->max_nuc + 63) / 64, 1, ns), dim3(64), 0, gs[k], ak,
                                        it);
                 const bool start = seg == nseg - 1 && it == t_hi - 1;
-                if (start) HIP_TRY(stagger_in(p, k, gs));
-                launch_adj<T, FD>(2, p->tz[2], a, ns, gs[k], it);
-                if (start) HIP_TRY(stagger_out(p, k, G, gs));
-                launch_adj<T, FD>(3, p->tz[3], a, ns, gs[k], it);
+                if (start) {
+                    h.wait_prev(k);
+                    HIP_TRY(stagger_in(p, k, gs));
+                }
+                launch_adj<T, FD>(2, p->tz[2], ak, ns, gs[k], it);
+                if (start) {
+                    HIP_TRY(stagger_out(p, k, G, gs));
+                    h.mark(k);
+                }
+                launch_adj<T, FD>(3, p->tz[3], ak, ns, gs[k], it);
             }
         }
-    }
-    HIP_TRY(hipGetLastError());
+        HIP_TRY(hipGetLastError());
+        return 0;
+    });
+    if (rc) return rc;
     HIP_TRY(join_groups(p, G, st));
     return 0;
 }
